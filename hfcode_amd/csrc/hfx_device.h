@@ -1,0 +1,111 @@
+// Device-side records and helpers shared by every hfx kernel (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HFX_NT 256                 // threads per workgroup of the class kernels
+#define HFX_INTTHRESH 1.0e-10      // element cut of the diagonal builds (molprops.f90:162)
+#define HFX_BOYS_ROWS 449          // Boys grid x = 0, 1/16, ..., 28
+#define HFX_BOYS_COLS 16           // F_0 .. F_15 per grid point
+#define HFX_BOYS_STEP 16.0         // grid points per unit x
+#define HFX_BOYS_XMAX 28.0         // above: asymptotic form (boys.f90:22-24)
+
+namespace hfx {
+
+// One shell pair, in the orientation of the reference pair lists
+// (precalc_kabs.f90:38-90): shell a is the first listed shell.
+struct PairRec {
+  double ax, ay, az, bx, by, bz;   // centres (bohr)
+  int sha, shb;                    // shells, 0-based
+  int aoa, aob;                    // first AO, 0-based
+  int p0;                          // first primitive pair record
+  int np;                          // primitive pairs that pass the negab test
+  int npall;                       // all primitive pairs (negab ones follow)
+  int rank;                        // position in the Schwarz order of its type
+};
+
+// One primitive pair (precalc_kabs.f90:94-115): p = za + zb, 1/p, the
+// product centre P, K_ab = sqrt(2) pi^(5/4) exp(-q R_AB^2) / p * d_a d_b, and
+// the L-shell p/s coefficient ratios of the two primitives (1 otherwise).
+struct PrimRec {
+  double p, pinv, Px, Py, Pz, K, wa, wb;
+};
+
+// Per-shell data the density-maximum kernel needs.
+struct ShellRec {
+  int type;      // 1 s, 2 p, 3 d, 12 L
+  int ao;        // first AO, 0-based
+  int ncomp;     // 1, 3, 5, 4
+  int pad;
+};
+
+struct ClassArgs {
+  const int2* list;        // surviving quartets (bra pair, ket pair)
+  const int* count;        // number of entries in list
+  const PairRec* pairs;
+  const PrimRec* prims;
+  const double* P;         // density, nb x nb
+  double* X;               // accumulator; F += X + X^T at the end
+  const double* boys64;    // HFX_BOYS_ROWS x HFX_BOYS_COLS
+  const float* boys32;
+  double* blocks;          // optional: dump of every block (625 per quartet)
+  double* schw_out;        // optional: sqrt(max|(ab|ab)|) per bra pair
+  double thr;
+  int nb;
+  int diag;                // 0 off-diagonal build, 1 bra==ket build
+};
+
+template <typename R> struct BoysTerms;
+template <> struct BoysTerms<double> { static constexpr int K = 8; };
+template <> struct BoysTerms<float> { static constexpr int K = 5; };
+
+template <typename R> __device__ inline const R* boys_src(const ClassArgs& a);
+template <> __device__ inline const double* boys_src<double>(const ClassArgs& a) { return a.boys64; }
+template <> __device__ inline const float* boys_src<float>(const ClassArgs& a) { return a.boys32; }
+
+// Copy columns 0..ncol-1 of the Boys table into LDS (row-major, ncol wide).
+template <typename R>
+__device__ inline void stage_boys(R* tab, const ClassArgs& a, int ncol) {
+  const R* src = boys_src<R>(a);
+  const int n = HFX_BOYS_ROWS * ncol;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int r = i / ncol, c = i - r * ncol;
+    tab[i] = src[r * HFX_BOYS_COLS + c];
+  }
+  __syncthreads();
+}
+
+// Boys F_m(T), m = 0..M.
+//  T > 28: F_0 = sqrt(pi/T)/2 and upward recursion without the exp term, the
+//          long-range branch of the reference (boys.f90:22-24, 117-123);
+//  else:   Taylor series about the nearest grid point x_i,
+//          F_m(T) = sum_k F_{m+k}(x_i) (x_i - T)^k / k!, using dF_m/dx = -F_{m+1};
+//          K terms give < 1e-16 (double) / 1e-10 (float) relative error for
+//          |x_i - T| <= 1/32.
+template <typename R, int M>
+__device__ __forceinline__ void boys(const R* __restrict__ tab, int ncol, R T, R* F) {
+  constexpr int K = BoysTerms<R>::K;
+  if (T > (R)HFX_BOYS_XMAX) {
+    const R t = (R)1 / T;
+    F[0] = (R)0.88622692545275801365 * sqrt(t);
+#pragma unroll
+    for (int m = 1; m <= M; ++m) F[m] = F[m - 1] * ((R)m - (R)0.5) * t;
+  } else {
+    const int i = (int)(T * (R)HFX_BOYS_STEP + (R)0.5);
+    const R d = (R)i * (R)(1.0 / HFX_BOYS_STEP) - T;
+    R c[K];
+    c[0] = 1;
+#pragma unroll
+    for (int k = 1; k < K; ++k) c[k] = c[k - 1] * d * (R)(1.0 / k);
+    const R* row = tab + i * ncol;
+#pragma unroll
+    for (int m = 0; m <= M; ++m) {
+      R s = row[m + K - 1] * c[K - 1];
+#pragma unroll
+      for (int k = K - 2; k >= 0; --k) s = fma(row[m + k], c[k], s);
+      F[m] = s;
+    }
+  }
+}
+
+}  // namespace hfx

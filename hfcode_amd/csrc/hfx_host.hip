@@ -1,0 +1,735 @@
+// Host side of the hfx C ABI (include/hfx.h): context set-up, primitive-pair
+// data, Schwarz lists, and the launch sequence of one Fock build.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hfx.h"
+#include "gen/eri_dispatch.h"
+#include "hfx_kernels.h"
+
+using namespace hfx;
+
+namespace {
+
+const char* kPairNames[HFX_NPAIRTYPES] = {"ss", "sp", "sd", "sl", "pp", "pd", "pl", "dd", "dl", "ll"};
+const char* kClassNames[HFX_NCLASSES] = {
+    "ssss", "sssp", "sssd", "sssl", "sspp", "sspd", "sspl", "ssdd", "ssdl", "ssll", "spsp", "spsd",
+    "spsl", "sppp", "sppd", "sppl", "spdd", "spll", "sdsd", "sdsl", "sdpp", "sdpd", "sddd", "sddl",
+    "sdll", "slsl", "slpp", "slpl", "sldd", "sldl", "slll", "pppp", "pppd", "pppl", "ppdd", "ppll",
+    "pdpd", "pddd", "plpl", "plll", "dddd", "dddl", "ddll", "dldl", "dlll", "llll"};
+// same-type class per pair type, used by the diagonal builds
+const int kDiagClass[HFX_NPAIRTYPES] = {0, 10, 18, 25, 31, 36, 38, 40, 43, 45};
+
+const double kShthresh = 1.0e-11;  // shthresh = intthresh/10 (molprops.f90:163)
+const double kNegab = 1.0e-12;     // extrathresh = intthresh*0.01 (precalc_kabs.f90:13)
+
+int letter(int shtyp) {  // canonical pair-name letter order s < p < d < L
+  switch (shtyp) {
+    case 1: return 0;
+    case 2: return 1;
+    case 3: return 2;
+    case 12: return 3;
+  }
+  return -1;
+}
+const char kLetters[4] = {'s', 'p', 'd', 'l'};
+
+int pair_type_index(char a, char b) {
+  for (int t = 0; t < HFX_NPAIRTYPES; ++t)
+    if (kPairNames[t][0] == a && kPairNames[t][1] == b) return t;
+  return -1;
+}
+
+int ncomp_of(int shtyp) { return shtyp == 1 ? 1 : shtyp == 2 ? 3 : shtyp == 3 ? 5 : 4; }
+
+struct Shell {
+  int type, nprim, ao;
+  double x[3];
+  double zeta[16], djk[16], djkl[16];
+};
+
+// Boys F_j(x) by the series F_j(x) = e^-x sum_k (2x)^k / ((2j+1)(2j+3)...(2j+2k+1)),
+// in long double; converges for all x on the grid (x <= 28).
+void boys_table(std::vector<double>& t64, std::vector<float>& t32) {
+  t64.assign(HFX_BOYS_ROWS * HFX_BOYS_COLS, 0.0);
+  t32.assign(HFX_BOYS_ROWS * HFX_BOYS_COLS, 0.0f);
+  for (int i = 0; i < HFX_BOYS_ROWS; ++i) {
+    const long double x = (long double)i / (long double)HFX_BOYS_STEP;
+    for (int j = 0; j < HFX_BOYS_COLS; ++j) {
+      long double term = 1.0L / (2 * j + 1), sum = term;
+      for (int k = 1; k < 400; ++k) {
+        term *= 2.0L * x / (long double)(2 * j + 2 * k + 1);
+        sum += term;
+        if (term < sum * 1e-21L) break;
+      }
+      const long double v = expl(-x) * sum;
+      t64[i * HFX_BOYS_COLS + j] = (double)v;
+      t32[i * HFX_BOYS_COLS + j] = (float)v;
+    }
+  }
+}
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+  if (n == 0) n = 1;
+  if (hipMalloc((void**)p, n * sizeof(T)) != hipSuccess) return HFX_E_NOMEM;
+  return HFX_OK;
+}
+
+}  // namespace
+
+struct ClassPlan {
+  int cls = 0, t1 = 0, t2 = 0, same = 0;
+  long long cap = 0;           // maximum number of quartets
+  int2* list64 = nullptr;
+  int2* list32 = nullptr;
+  double* blocks64 = nullptr;  // debug
+  double* blocks32 = nullptr;
+};
+
+struct hfx_ctx {
+  int nb = 0, nsh = 0, dev = 0, ncu = 0;
+  std::vector<Shell> shells;
+  int npair[HFX_NPAIRTYPES] = {0};
+  int offset[HFX_NPAIRTYPES] = {0};
+  std::vector<int> prs[HFX_NPAIRTYPES];         // 2 per pair, 0-based shells
+  std::vector<double> schw[HFX_NPAIRTYPES];     // per pair (list order)
+  std::vector<int> order[HFX_NPAIRTYPES];       // 0-based
+  int last[HFX_NPAIRTYPES] = {0};
+  bool have_schwarz = false;
+  std::vector<PairRec> pairs;
+  std::vector<PrimRec> prims;
+  std::vector<ClassPlan> classes;
+  int totpairs = 0;
+  double split = 0.0;
+  int debug_blocks = 0;
+  int last_kind = 0;  // 0 none, 1 non-diag, 2 diag
+  // device
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  PairRec* d_pairs = nullptr;
+  PrimRec* d_prims = nullptr;
+  ShellRec* d_shells = nullptr;
+  double* d_boys64 = nullptr;
+  float* d_boys32 = nullptr;
+  double* d_P = nullptr;
+  double* d_F = nullptr;
+  double* d_X = nullptr;
+  double* d_maxdm = nullptr;
+  double* d_schw_out = nullptr;   // per global pair
+  int* d_act[HFX_NPAIRTYPES] = {nullptr};        // active global pair indices (rank order)
+  double* d_act_schw[HFX_NPAIRTYPES] = {nullptr};
+  double* d_pair_schw[HFX_NPAIRTYPES] = {nullptr};  // Schwarz per pair (list order)
+  int2* d_diag_list[HFX_NPAIRTYPES] = {nullptr};
+  int* d_counts = nullptr;                     // [2][HFX_NCLASSES]
+  unsigned long long* d_stats = nullptr;       // [2][HFX_NCLASSES][3]
+};
+
+#define HIPOK(x)                                                              \
+  do {                                                                        \
+    hipError_t e_ = (x);                                                      \
+    if (e_ != hipSuccess) {                                                   \
+      fprintf(stderr, "hfx: %s failed: %s\n", #x, hipGetErrorString(e_));    \
+      return HFX_E_HIP;                                                       \
+    }                                                                         \
+  } while (0)
+
+namespace {
+
+int grid_for(long long n, int nt, int maxblocks) {
+  long long g = (n + nt - 1) / nt;
+  if (g < 1) g = 1;
+  if (g > maxblocks) g = maxblocks;
+  return (int)g;
+}
+
+// Active (Schwarz-truncated) lists on the device, from schw/order/last.
+int upload_active(hfx_ctx* c) {
+  for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
+    const int n = c->npair[t];
+    if (n == 0) continue;
+    std::vector<int> act(c->last[t]);
+    std::vector<double> asc(c->last[t]);
+    for (int k = 0; k < c->last[t]; ++k) {
+      const int pi = c->order[t][k];
+      act[k] = c->offset[t] + pi;
+      asc[k] = c->schw[t][pi];
+      c->pairs[act[k]].rank = k;
+    }
+    if (c->d_act[t]) { hipFree(c->d_act[t]); c->d_act[t] = nullptr; }
+    if (c->d_act_schw[t]) { hipFree(c->d_act_schw[t]); c->d_act_schw[t] = nullptr; }
+    if (dalloc(&c->d_act[t], act.size()) || dalloc(&c->d_act_schw[t], asc.size())) return HFX_E_NOMEM;
+    HIPOK(hipMemcpy(c->d_act[t], act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(c->d_act_schw[t], asc.data(), asc.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(c->d_pair_schw[t], c->schw[t].data(), n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  HIPOK(hipMemcpy(c->d_pairs, c->pairs.data(), c->pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice));
+  // class plans and their work lists
+  for (auto& cp : c->classes) {
+    if (cp.list64) { hipFree(cp.list64); cp.list64 = nullptr; }
+    if (cp.list32) { hipFree(cp.list32); cp.list32 = nullptr; }
+    const long long nb = c->last[cp.t1], nk = c->last[cp.t2];
+    cp.cap = cp.same ? nb * (nb - 1) / 2 : nb * nk;
+    if (cp.cap > 0 && dalloc(&cp.list64, (size_t)cp.cap)) return HFX_E_NOMEM;
+  }
+  c->have_schwarz = true;
+  return HFX_OK;
+}
+
+int ensure_mixed_lists(hfx_ctx* c) {
+  for (auto& cp : c->classes)
+    if (!cp.list32 && cp.cap > 0 && dalloc(&cp.list32, (size_t)cp.cap)) return HFX_E_NOMEM;
+  return HFX_OK;
+}
+
+ClassArgs base_args(hfx_ctx* c, const double* dP, double thr) {
+  ClassArgs a;
+  memset(&a, 0, sizeof(a));
+  a.pairs = c->d_pairs;
+  a.prims = c->d_prims;
+  a.P = dP;
+  a.X = c->d_X;
+  a.boys64 = c->d_boys64;
+  a.boys32 = c->d_boys32;
+  a.thr = thr;
+  a.nb = c->nb;
+  return a;
+}
+
+int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long long cap) {
+  const ClassKernels& k = kClassKernels[cls];
+  const void* fn = prec == 1 ? k.f32 : (jonly ? k.f64j : k.f64);
+  const int K = prec == 1 ? 5 : 8;
+  const size_t shm = (size_t)HFX_BOYS_ROWS * (k.M + K) * (prec == 1 ? sizeof(float) : sizeof(double));
+  const int grid = grid_for(cap, HFX_NT, c->ncu * 8);
+  void* args[] = {&a};
+  HIPOK(hipLaunchKernel(fn, dim3(grid), dim3(HFX_NT), args, shm, c->stream));
+  return HFX_OK;
+}
+
+int zero_build_state(hfx_ctx* c) {
+  const size_t nx = (size_t)c->nb * c->nb;
+  k_fill_f64<<<grid_for((long long)nx, 256, 2048), 256, 0, c->stream>>>(c->d_X, nx, 0.0);
+  k_fill_i32<<<1, 256, 0, c->stream>>>(c->d_counts, 2 * HFX_NCLASSES, 0);
+  k_fill_u64<<<1, 256, 0, c->stream>>>(c->d_stats, 2 * HFX_NCLASSES * 3);
+  HIPOK(hipGetLastError());
+  return HFX_OK;
+}
+
+int launch_maxdm(hfx_ctx* c, const double* dP) {
+  const long long n = (long long)c->nsh * c->nsh;
+  k_maxdm<<<grid_for(n, 256, 4096), 256, 0, c->stream>>>(dP, c->nb, c->d_shells, c->nsh, c->d_maxdm);
+  HIPOK(hipGetLastError());
+  return HFX_OK;
+}
+
+int alloc_debug_blocks(hfx_ctx* c, ClassPlan& cp, long long cap, int prec) {
+  double** b = prec ? &cp.blocks32 : &cp.blocks64;
+  if (*b) { hipFree(*b); *b = nullptr; }
+  if (cap > 0 && dalloc(b, (size_t)cap * 625)) return HFX_E_NOMEM;
+  return HFX_OK;
+}
+
+// Off-diagonal build on device arrays: F += G(P).
+int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode) {
+  if (!c->have_schwarz) return HFX_E_STATE;
+  const bool jonly = mode == HFX_J_ONLY;
+  const bool mixed = c->split > 0.0 && !jonly;
+  if (mixed) {
+    int r = ensure_mixed_lists(c);
+    if (r) return r;
+  }
+  int r;
+  HIPOK(hipEventRecord(c->ev0, c->stream));
+  if ((r = zero_build_state(c))) return r;
+  if ((r = launch_maxdm(c, dP))) return r;
+  for (auto& cp : c->classes) {
+    if (cp.cap <= 0) continue;
+    ScreenArgs s;
+    memset(&s, 0, sizeof(s));
+    s.bra = c->d_act[cp.t1];
+    s.ket = c->d_act[cp.t2];
+    s.schw_bra = c->d_act_schw[cp.t1];
+    s.schw_ket = c->d_act_schw[cp.t2];
+    s.pairs = c->d_pairs;
+    s.maxdm = c->d_maxdm;
+    s.list64 = cp.list64;
+    s.list32 = mixed ? cp.list32 : nullptr;
+    s.count64 = c->d_counts + cp.cls;
+    s.count32 = c->d_counts + HFX_NCLASSES + cp.cls;
+    s.stats64 = c->d_stats + 3 * cp.cls;
+    s.stats32 = c->d_stats + 3 * (HFX_NCLASSES + cp.cls);
+    s.thr = thr;
+    s.split = mixed ? c->split : 0.0;
+    s.nbra = c->last[cp.t1];
+    s.nket = c->last[cp.t2];
+    s.same = cp.same;
+    s.jonly = jonly;
+    s.nsh = c->nsh;
+    s.all_prims = kClassKernels[cp.cls].all_prims;
+    const long long n = (long long)s.nbra * s.nket;
+    k_screen<<<grid_for(n, 256, c->ncu * 16), 256, 0, c->stream>>>(s);
+    HIPOK(hipGetLastError());
+  }
+  for (auto& cp : c->classes) {
+    if (cp.cap <= 0) continue;
+    for (int prec = 0; prec < (mixed ? 2 : 1); ++prec) {
+      ClassArgs a = base_args(c, dP, thr);
+      a.list = prec ? cp.list32 : cp.list64;
+      a.count = c->d_counts + prec * HFX_NCLASSES + cp.cls;
+      a.diag = 0;
+      if (c->debug_blocks) {
+        if ((r = alloc_debug_blocks(c, cp, cp.cap, prec))) return r;
+        a.blocks = prec ? cp.blocks32 : cp.blocks64;
+      }
+      if ((r = launch_class(c, cp.cls, prec, jonly, a, cp.cap))) return r;
+    }
+  }
+  k_add_sym<<<grid_for((long long)c->nb * c->nb, 256, 2048), 256, 0, c->stream>>>(c->d_X, dF, c->nb);
+  HIPOK(hipGetLastError());
+  HIPOK(hipEventRecord(c->ev1, c->stream));
+  c->last_kind = 1;
+  return HFX_OK;
+}
+
+// Diagonal-pair build (fock_build_diag, or fock_build_initial_diag when
+// initial != 0: no density test, Schwarz values written out).
+int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initial) {
+  int r;
+  HIPOK(hipEventRecord(c->ev0, c->stream));
+  if ((r = zero_build_state(c))) return r;
+  if ((r = launch_maxdm(c, dP))) return r;
+  for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
+    const int n = c->npair[t];
+    if (n == 0) continue;
+    const int cls = kDiagClass[t];
+    DiagScreenArgs s;
+    memset(&s, 0, sizeof(s));
+    s.pairs = c->d_pairs;
+    s.schw = c->d_pair_schw[t];
+    s.maxdm = c->d_maxdm;
+    s.list = c->d_diag_list[t];
+    s.count = c->d_counts + cls;
+    s.stats = c->d_stats + 3 * cls;
+    s.thr = thr;
+    s.first = c->offset[t];
+    s.npair = n;
+    s.nsh = c->nsh;
+    s.check = initial ? 0 : 1;
+    s.all_prims = kClassKernels[cls].all_prims;
+    k_screen_diag<<<grid_for(n, 256, 1024), 256, 0, c->stream>>>(s);
+    HIPOK(hipGetLastError());
+    ClassArgs a = base_args(c, dP, thr);
+    a.list = c->d_diag_list[t];
+    a.count = c->d_counts + cls;
+    a.diag = 1;
+    a.schw_out = initial ? c->d_schw_out : nullptr;
+    if (c->debug_blocks) {
+      ClassPlan* cp = nullptr;
+      for (auto& x : c->classes)
+        if (x.cls == cls) cp = &x;
+      if (cp) {
+        if ((r = alloc_debug_blocks(c, *cp, n, 0))) return r;
+        a.blocks = cp->blocks64;
+      }
+    }
+    if ((r = launch_class(c, cls, 0, false, a, n))) return r;
+  }
+  k_add_sym<<<grid_for((long long)c->nb * c->nb, 256, 2048), 256, 0, c->stream>>>(c->d_X, dF, c->nb);
+  HIPOK(hipGetLastError());
+  HIPOK(hipEventRecord(c->ev1, c->stream));
+  c->last_kind = 2;
+  return HFX_OK;
+}
+
+// Descending Schwarz order and truncation index, as index_sort / find_threshold
+// (index_sort.f90:1-119) define them: `last` is the position of the first
+// pair below shthresh (that pair included), or n when none is below.
+void sort_schwarz(hfx_ctx* c, int t) {
+  const int n = c->npair[t];
+  std::vector<int>& o = c->order[t];
+  o.resize(n);
+  for (int i = 0; i < n; ++i) o[i] = i;
+  const std::vector<double>& s = c->schw[t];
+  std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return s[a] > s[b]; });
+  int last = n;
+  if (n > 0 && s[o[0]] < kShthresh) {
+    last = 1;
+  } else if (n > 0 && s[o[n - 1]] > kShthresh) {
+    last = n;
+  } else {
+    for (int i = 0; i < n; ++i)
+      if (s[o[i]] < kShthresh) { last = i + 1; break; }
+  }
+  c->last[t] = last;
+}
+
+int copy_fock_host(hfx_ctx* c, const double* P, double* F, int nb) {
+  const size_t bytes = (size_t)nb * nb * sizeof(double);
+  HIPOK(hipMemcpyAsync(c->d_P, P, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPOK(hipMemcpyAsync(c->d_F, F, bytes, hipMemcpyHostToDevice, c->stream));
+  return HFX_OK;
+}
+
+int fetch_fock_host(hfx_ctx* c, double* F, int nb) {
+  const size_t bytes = (size_t)nb * nb * sizeof(double);
+  HIPOK(hipMemcpyAsync(F, c->d_F, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPOK(hipStreamSynchronize(c->stream));
+  return HFX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hfx_version(void) { return "hfx-mi355x 0.1 (gfx950)"; }
+
+int hfx_ctx_create(const hfx_mol_desc* d, hfx_ctx** out) {
+  if (!d || !out || d->nb <= 0 || d->nsh <= 0 || !d->shtyp || !d->nzet || !d->aosh || !d->xsh || !d->zeta ||
+      !d->djk || d->maxprim <= 0 || d->maxprim > 16)
+    return HFX_E_ARG;
+  *out = nullptr;
+  hfx_ctx* c = new hfx_ctx();
+  c->nb = d->nb;
+  c->nsh = d->nsh;
+  bool hasP = false, hasD = false, hasL = false;
+  c->shells.resize(d->nsh);
+  for (int i = 0; i < d->nsh; ++i) {
+    Shell& s = c->shells[i];
+    s.type = d->shtyp[i];
+    if (letter(s.type) < 0) { delete c; return HFX_E_ARG; }
+    hasP |= s.type == 2; hasD |= s.type == 3; hasL |= s.type == 12;
+    s.nprim = d->nzet[i];
+    if (s.nprim <= 0 || s.nprim > d->maxprim) { delete c; return HFX_E_UNSUPPORTED; }
+    s.ao = d->aosh[i] - 1;
+    if (s.ao < 0 || s.ao + ncomp_of(s.type) > d->nb) { delete c; return HFX_E_ARG; }
+    for (int k = 0; k < 3; ++k) s.x[k] = d->xsh[3 * i + k];
+    for (int k = 0; k < s.nprim; ++k) {
+      s.zeta[k] = d->zeta[(size_t)d->maxprim * i + k];
+      s.djk[k] = d->djk[(size_t)d->maxprim * i + k];
+      s.djkl[k] = (s.type == 12 && d->djkl) ? d->djkl[(size_t)d->maxprim * i + k] : 1.0;
+    }
+  }
+  if (hasP && hasD && hasL) { delete c; return HFX_E_UNSUPPORTED; }  // spread_basis.f90:174-177
+
+  // pair lists
+  const double rt2pi54 = std::sqrt(2.0 * M_PI * M_PI * std::sqrt(M_PI));  // nrtype.f90:12-13
+  bool schwarz_given = true;
+  int tot = 0;
+  for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
+    const int n = d->npair[t];
+    c->npair[t] = n;
+    c->offset[t] = tot;
+    tot += n;
+    if (n == 0) continue;
+    if (!d->prs[t]) { delete c; return HFX_E_ARG; }
+    c->prs[t].resize(2 * n);
+    for (int k = 0; k < n; ++k) {
+      const int s1 = d->prs[t][2 * k] - 1, s2 = d->prs[t][2 * k + 1] - 1;
+      if (s1 < 0 || s2 < 0 || s1 >= d->nsh || s2 >= d->nsh) { delete c; return HFX_E_ARG; }
+      if (kLetters[letter(c->shells[s1].type)] != kPairNames[t][0] ||
+          kLetters[letter(c->shells[s2].type)] != kPairNames[t][1]) { delete c; return HFX_E_ARG; }
+      c->prs[t][2 * k] = s1;
+      c->prs[t][2 * k + 1] = s2;
+    }
+    if (d->schw[t] && d->order[t]) {
+      c->schw[t].assign(d->schw[t], d->schw[t] + n);
+      c->order[t].resize(n);
+      for (int k = 0; k < n; ++k) {
+        c->order[t][k] = d->order[t][k] - 1;
+        if (c->order[t][k] < 0 || c->order[t][k] >= n) { delete c; return HFX_E_ARG; }
+      }
+      c->last[t] = d->last[t];
+      if (c->last[t] < 0 || c->last[t] > n) { delete c; return HFX_E_ARG; }
+    } else {
+      schwarz_given = false;
+      c->schw[t].assign(n, 0.0);
+      c->order[t].resize(n);
+      for (int k = 0; k < n; ++k) c->order[t][k] = k;
+      c->last[t] = 0;
+    }
+  }
+  c->totpairs = tot;
+  // primitive pairs, precalc_kabs.f90:94-115, with the negab ones moved last
+  c->pairs.resize(tot);
+  for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
+    for (int k = 0; k < c->npair[t]; ++k) {
+      const Shell& A = c->shells[c->prs[t][2 * k]];
+      const Shell& B = c->shells[c->prs[t][2 * k + 1]];
+      PairRec& pr = c->pairs[c->offset[t] + k];
+      pr.ax = A.x[0]; pr.ay = A.x[1]; pr.az = A.x[2];
+      pr.bx = B.x[0]; pr.by = B.x[1]; pr.bz = B.x[2];
+      pr.sha = c->prs[t][2 * k];
+      pr.shb = c->prs[t][2 * k + 1];
+      pr.aoa = A.ao;
+      pr.aob = B.ao;
+      pr.rank = -1;
+      const double dx = A.x[0] - B.x[0], dy = A.x[1] - B.x[1], dz = A.x[2] - B.x[2];
+      const double rab2 = dx * dx + dy * dy + dz * dz;
+      std::vector<PrimRec> keep, neg;
+      for (int za = 0; za < A.nprim; ++za) {
+        for (int zb = 0; zb < B.nprim; ++zb) {
+          PrimRec r;
+          const double a = A.zeta[za], b = B.zeta[zb];
+          r.p = a + b;
+          r.pinv = 1.0 / r.p;
+          const double q = a * b * r.pinv;
+          r.Px = (a * A.x[0] + b * B.x[0]) * r.pinv;
+          r.Py = (a * A.x[1] + b * B.x[1]) * r.pinv;
+          r.Pz = (a * A.x[2] + b * B.x[2]) * r.pinv;
+          const double dd = A.djk[za] * B.djk[zb];
+          r.K = rt2pi54 * std::exp(-q * rab2) * r.pinv * dd;
+          r.wa = A.type == 12 ? A.djkl[za] : 1.0;
+          r.wb = B.type == 12 ? B.djkl[zb] : 1.0;
+          if (std::fabs(r.K) * std::sqrt(r.pinv) < kNegab) neg.push_back(r); else keep.push_back(r);
+        }
+      }
+      pr.p0 = (int)c->prims.size();
+      pr.np = (int)keep.size();
+      pr.npall = (int)(keep.size() + neg.size());
+      c->prims.insert(c->prims.end(), keep.begin(), keep.end());
+      c->prims.insert(c->prims.end(), neg.begin(), neg.end());
+    }
+  }
+  // classes
+  for (int cl = 0; cl < HFX_NCLASSES; ++cl) {
+    ClassPlan cp;
+    cp.cls = cl;
+    cp.t1 = pair_type_index(kClassNames[cl][0], kClassNames[cl][1]);
+    cp.t2 = pair_type_index(kClassNames[cl][2], kClassNames[cl][3]);
+    cp.same = cp.t1 == cp.t2;
+    if (c->npair[cp.t1] == 0 || c->npair[cp.t2] == 0) continue;
+    c->classes.push_back(cp);
+  }
+
+  // device
+  int r = HFX_OK;
+  auto fail = [&](int code) { hfx_ctx_destroy(c); return code; };
+  if (hipGetDevice(&c->dev) != hipSuccess) return fail(HFX_E_HIP);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->dev) != hipSuccess) return fail(HFX_E_HIP);
+  c->ncu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(HFX_E_HIP);
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(HFX_E_HIP);
+  std::vector<ShellRec> sr(c->nsh);
+  for (int i = 0; i < c->nsh; ++i) sr[i] = ShellRec{c->shells[i].type, c->shells[i].ao, ncomp_of(c->shells[i].type), 0};
+  std::vector<double> b64;
+  std::vector<float> b32;
+  boys_table(b64, b32);
+  const size_t nb2 = (size_t)c->nb * c->nb;
+  if ((r = dalloc(&c->d_pairs, c->pairs.size())) || (r = dalloc(&c->d_prims, c->prims.size())) ||
+      (r = dalloc(&c->d_shells, sr.size())) || (r = dalloc(&c->d_boys64, b64.size())) ||
+      (r = dalloc(&c->d_boys32, b32.size())) || (r = dalloc(&c->d_P, nb2)) || (r = dalloc(&c->d_F, nb2)) ||
+      (r = dalloc(&c->d_X, nb2)) || (r = dalloc(&c->d_maxdm, (size_t)c->nsh * c->nsh)) ||
+      (r = dalloc(&c->d_schw_out, (size_t)tot)) || (r = dalloc(&c->d_counts, 2 * HFX_NCLASSES)) ||
+      (r = dalloc(&c->d_stats, 2 * HFX_NCLASSES * 3)))
+    return fail(r);
+  for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
+    if (c->npair[t] == 0) continue;
+    if ((r = dalloc(&c->d_diag_list[t], c->npair[t])) || (r = dalloc(&c->d_pair_schw[t], c->npair[t])))
+      return fail(r);
+  }
+  if (hipMemcpy(c->d_prims, c->prims.data(), c->prims.size() * sizeof(PrimRec), hipMemcpyHostToDevice) ||
+      hipMemcpy(c->d_shells, sr.data(), sr.size() * sizeof(ShellRec), hipMemcpyHostToDevice) ||
+      hipMemcpy(c->d_boys64, b64.data(), b64.size() * sizeof(double), hipMemcpyHostToDevice) ||
+      hipMemcpy(c->d_boys32, b32.data(), b32.size() * sizeof(float), hipMemcpyHostToDevice) ||
+      hipMemcpy(c->d_pairs, c->pairs.data(), c->pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice))
+    return fail(HFX_E_HIP);
+  if (schwarz_given) {
+    if ((r = upload_active(c))) return fail(r);
+  } else {
+    for (int t = 0; t < HFX_NPAIRTYPES; ++t)
+      if (c->npair[t] && hipMemcpy(c->d_pair_schw[t], c->schw[t].data(), c->npair[t] * sizeof(double),
+                                   hipMemcpyHostToDevice))
+        return fail(HFX_E_HIP);
+  }
+  *out = c;
+  return HFX_OK;
+}
+
+void hfx_ctx_destroy(hfx_ctx* c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto& cp : c->classes) {
+    hipFree(cp.list64); hipFree(cp.list32); hipFree(cp.blocks64); hipFree(cp.blocks32);
+  }
+  for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
+    hipFree(c->d_act[t]); hipFree(c->d_act_schw[t]); hipFree(c->d_pair_schw[t]); hipFree(c->d_diag_list[t]);
+  }
+  hipFree(c->d_pairs); hipFree(c->d_prims); hipFree(c->d_shells); hipFree(c->d_boys64); hipFree(c->d_boys32);
+  hipFree(c->d_P); hipFree(c->d_F); hipFree(c->d_X); hipFree(c->d_maxdm); hipFree(c->d_schw_out);
+  hipFree(c->d_counts); hipFree(c->d_stats);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int hfx_set_option(hfx_ctx* c, int key, double value) {
+  if (!c) return HFX_E_ARG;
+  if (key == HFX_OPT_MIXED_SPLIT) {
+    if (!(value >= 0.0)) return HFX_E_ARG;
+    c->split = value;
+    return HFX_OK;
+  }
+  if (key == HFX_OPT_DEBUG_BLOCKS) {
+    c->debug_blocks = value != 0.0;
+    return HFX_OK;
+  }
+  return HFX_E_ARG;
+}
+
+int hfx_fock_nd(hfx_ctx* c, const double* P, double* F, int nb, double thr, int mode) {
+  if (!c || !P || !F || nb != c->nb || (mode != HFX_JK && mode != HFX_J_ONLY)) return HFX_E_ARG;
+  if (!c->have_schwarz) return HFX_E_STATE;
+  int r;
+  if ((r = copy_fock_host(c, P, F, nb))) return r;
+  if ((r = build_nd(c, c->d_P, c->d_F, thr, mode))) return r;
+  return fetch_fock_host(c, F, nb);
+}
+
+int hfx_fock_diag(hfx_ctx* c, const double* P, double* F, int nb, double thr) {
+  if (!c || !P || !F || nb != c->nb) return HFX_E_ARG;
+  if (!c->have_schwarz) return HFX_E_STATE;
+  int r;
+  if ((r = copy_fock_host(c, P, F, nb))) return r;
+  if ((r = build_diag(c, c->d_P, c->d_F, thr, false))) return r;
+  return fetch_fock_host(c, F, nb);
+}
+
+int hfx_fock_initial_diag(hfx_ctx* c, const double* P, double* F, int nb, double thr) {
+  if (!c || !P || !F || nb != c->nb) return HFX_E_ARG;
+  int r;
+  if ((r = copy_fock_host(c, P, F, nb))) return r;
+  if ((r = build_diag(c, c->d_P, c->d_F, thr, true))) return r;
+  if ((r = fetch_fock_host(c, F, nb))) return r;
+  if (!c->have_schwarz) {
+    std::vector<double> s(c->totpairs);
+    HIPOK(hipMemcpy(s.data(), c->d_schw_out, s.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
+      for (int k = 0; k < c->npair[t]; ++k) c->schw[t][k] = s[c->offset[t] + k];
+      sort_schwarz(c, t);
+    }
+    if ((r = upload_active(c))) return r;
+  }
+  return HFX_OK;
+}
+
+int hfx_fock_nd_device(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, void* stream) {
+  if (!c || !dP || !dF || (mode != HFX_JK && mode != HFX_J_ONLY)) return HFX_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  if (s && s != c->stream) {
+    hipEvent_t e;
+    HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPOK(hipEventRecord(e, s));
+    HIPOK(hipStreamWaitEvent(c->stream, e, 0));
+    hipEventDestroy(e);
+  }
+  int r = build_nd(c, dP, dF, thr, mode);
+  if (r) return r;
+  if (s && s != c->stream) HIPOK(hipStreamWaitEvent(s, c->ev1, 0));
+  return HFX_OK;
+}
+
+int hfx_fock_diag_device(hfx_ctx* c, const double* dP, double* dF, double thr, void* stream) {
+  if (!c || !dP || !dF) return HFX_E_ARG;
+  if (!c->have_schwarz) return HFX_E_STATE;
+  hipStream_t s = (hipStream_t)stream;
+  if (s && s != c->stream) {
+    hipEvent_t e;
+    HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPOK(hipEventRecord(e, s));
+    HIPOK(hipStreamWaitEvent(c->stream, e, 0));
+    hipEventDestroy(e);
+  }
+  int r = build_diag(c, dP, dF, thr, false);
+  if (r) return r;
+  if (s && s != c->stream) HIPOK(hipStreamWaitEvent(s, c->ev1, 0));
+  return HFX_OK;
+}
+
+int hfx_get_stats(hfx_ctx* c, hfx_stats* out) {
+  if (!c || !out) return HFX_E_ARG;
+  memset(out, 0, sizeof(*out));
+  if (!c->last_kind) return HFX_OK;
+  HIPOK(hipStreamSynchronize(c->stream));
+  std::vector<unsigned long long> st(2 * HFX_NCLASSES * 3);
+  HIPOK(hipMemcpy(st.data(), c->d_stats, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  for (int cl = 0; cl < HFX_NCLASSES; ++cl) {
+    const ClassKernels& k = kClassKernels[cl];
+    for (int p = 0; p < 2; ++p) {
+      const unsigned long long* s = &st[3 * (p * HFX_NCLASSES + cl)];
+      out->quartets[cl][p] = (long long)s[0];
+      out->prim_quartets[cl][p] = (long long)s[1];
+      out->bra_prims[cl][p] = (long long)s[2];
+      const double f = (double)s[1] * (p ? k.flops_prim_fp32 : k.flops_prim) + (double)s[2] * k.flops_braprim;
+      if (p) {
+        out->flops_fp32 += f;
+        out->flops_fp64 += (double)s[0] * k.flops_quartet;  // digestion is FP64
+      } else {
+        out->flops_fp64 += f + (double)s[0] * k.flops_quartet;
+      }
+    }
+  }
+  float ms = 0.f;
+  HIPOK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  out->ms_total = ms;
+  return HFX_OK;
+}
+
+int hfx_get_schwarz(hfx_ctx* c, int t, double* schw, int* order, int* last) {
+  if (!c || t < 0 || t >= HFX_NPAIRTYPES) return HFX_E_ARG;
+  if (!c->have_schwarz) return HFX_E_STATE;
+  for (int k = 0; k < c->npair[t]; ++k) {
+    if (schw) schw[k] = c->schw[t][k];
+    if (order) order[k] = c->order[t][k] + 1;
+  }
+  if (last) *last = c->last[t];
+  return HFX_OK;
+}
+
+long long hfx_debug_blocks(hfx_ctx* c, int cls, int prec, int* quartets, double* blocks) {
+  if (!c || cls < 0 || cls >= HFX_NCLASSES || prec < 0 || prec > 1) return HFX_E_ARG;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return HFX_E_HIP;
+  ClassPlan* cp = nullptr;
+  for (auto& x : c->classes)
+    if (x.cls == cls) cp = &x;
+  if (!cp) return 0;
+  int n = 0;
+  if (hipMemcpy(&n, c->d_counts + prec * HFX_NCLASSES + cls, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    return HFX_E_HIP;
+  if (n <= 0) return 0;
+  const int2* list = nullptr;
+  if (c->last_kind == 2) {
+    for (int t = 0; t < HFX_NPAIRTYPES; ++t)
+      if (kDiagClass[t] == cls) list = c->d_diag_list[t];
+  } else {
+    list = prec ? cp->list32 : cp->list64;
+  }
+  double* b = prec ? cp->blocks32 : cp->blocks64;
+  if (quartets && list && hipMemcpy(quartets, list, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost) != hipSuccess)
+    return HFX_E_HIP;
+  if (blocks && b && hipMemcpy(blocks, b, (size_t)n * 625 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return HFX_E_HIP;
+  return n;
+}
+
+int hfx_pair_info(hfx_ctx* c, int g, int* type, int* index) {
+  if (!c || g < 0 || g >= c->totpairs) return HFX_E_ARG;
+  for (int t = HFX_NPAIRTYPES - 1; t >= 0; --t) {
+    if (c->npair[t] && g >= c->offset[t]) {
+      if (type) *type = t;
+      if (index) *index = g - c->offset[t] + 1;
+      return HFX_OK;
+    }
+  }
+  return HFX_E_ARG;
+}
+
+}  // extern "C"

@@ -1,0 +1,44 @@
+// Declarations of the support kernels (hfx_kernels.hip) and their arguments.
+#pragma once
+#include "hfx_device.h"
+
+namespace hfx {
+
+struct ScreenArgs {
+  const int* bra;              // global pair index per bra list position
+  const int* ket;
+  const double* schw_bra;      // Schwarz value per list position
+  const double* schw_ket;
+  const PairRec* pairs;
+  const double* maxdm;         // nsh x nsh
+  int2* list64;
+  int2* list32;
+  int* count64;
+  int* count32;
+  unsigned long long* stats64; // [quartets, primitive quartets, bra primitives]
+  unsigned long long* stats32;
+  double thr;
+  double split;                // > 0: bound < split -> FP32 list
+  int nbra, nket, same, jonly, nsh, all_prims;
+};
+
+struct DiagScreenArgs {
+  const PairRec* pairs;
+  const double* schw;          // Schwarz value per pair of the type (list order)
+  const double* maxdm;
+  int2* list;
+  int* count;
+  unsigned long long* stats;
+  double thr;
+  int first, npair, nsh, check, all_prims;
+};
+
+__global__ void k_fill_f64(double* x, size_t n, double v);
+__global__ void k_fill_i32(int* x, int n, int v);
+__global__ void k_fill_u64(unsigned long long* x, int n);
+__global__ void k_maxdm(const double* P, int nb, const ShellRec* sh, int nsh, double* maxdm);
+__global__ void k_screen(const ScreenArgs s);
+__global__ void k_screen_diag(const DiagScreenArgs s);
+__global__ void k_add_sym(const double* X, double* F, int nb);
+
+}  // namespace hfx

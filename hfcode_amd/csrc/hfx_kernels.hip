@@ -1,0 +1,174 @@
+// Support kernels of the Fock build: fills, density maxima, screening and
+// work-list compaction, and the final symmetrised add into F.
+#include "hfx_kernels.h"
+
+namespace hfx {
+
+__global__ void k_fill_f64(double* x, size_t n, double v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) x[i] = v;
+}
+
+__global__ void k_fill_i32(int* x, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* x, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = 0ull;
+}
+
+// maxdm(a,b) = max |P| over the AO block of shell pair (a,b), stored as a
+// symmetric nsh x nsh matrix.  The block is the one evaluate_maxdm reads
+// (evaluate_maxdm.f90:1-101): rows = AOs of the first shell of the pair in
+// list order (s before p before d before L), columns = AOs of the second.
+// For a (d, L) pair the reference reads rows ao_d..ao_d+3 and columns
+// ao_L..ao_L+4 (evaluate_maxdm.f90:82-90): 4 of the 5 d functions and the AO
+// after the L shell.  That block is reproduced so that screening decisions
+// match the reference exactly; a column past nb is clamped.
+__global__ void k_maxdm(const double* __restrict__ P, int nb, const ShellRec* __restrict__ sh, int nsh,
+                        double* __restrict__ maxdm) {
+  const long long n = (long long)nsh * nsh;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / nsh), j = (int)(idx - (long long)i * nsh);
+    if (j > i) continue;
+    ShellRec s1 = sh[i], s2 = sh[j];
+    // list orientation: canonical letter order s < p < d < L
+    auto rank = [](int t) { return t == 1 ? 0 : t == 2 ? 1 : t == 3 ? 2 : 3; };
+    if (rank(s2.type) < rank(s1.type)) { ShellRec t = s1; s1 = s2; s2 = t; }
+    int nr = s1.ncomp, nc = s2.ncomp;
+    if (s1.type == 3 && s2.type == 12) { nr = 4; nc = 5; }
+    double m = 0.0;
+    for (int r = 0; r < nr; ++r) {
+      const int ar = s1.ao + r;
+      for (int c = 0; c < nc; ++c) {
+        int ac = s2.ao + c;
+        if (ac >= nb) ac = nb - 1;
+        m = fmax(m, fabs(P[(size_t)ar * nb + ac]));
+      }
+    }
+    maxdm[(size_t)i * nsh + j] = m;
+    maxdm[(size_t)j * nsh + i] = m;
+  }
+}
+
+// Wave-aggregated append of (bra, ket) to a list: one atomic per wave.
+__device__ inline void wave_append(bool pred, int2 v, int2* list, int* count, unsigned long long* stats,
+                                   unsigned long long primq, unsigned long long brap) {
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)mask) - 1;
+  const int n = __popcll(mask);
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, n);
+  base = __shfl(base, leader);
+  if (pred) {
+    const int off = __popcll(mask & ((1ull << lane) - 1ull));
+    list[base + off] = v;
+  }
+  // statistics: wave sums of primitive quartets and bra primitive iterations
+  unsigned long long a = pred ? primq : 0ull, b = pred ? brap : 0ull;
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o);
+    b += __shfl_xor(b, o);
+  }
+  if (lane == leader) {
+    atomicAdd(&stats[0], (unsigned long long)n);
+    atomicAdd(&stats[1], a);
+    atomicAdd(&stats[2], b);
+  }
+}
+
+// Screening of one class (fock_build_non_diag.f90:63-67, e.g. :58-67):
+// bound = schw_bra * schw_ket * max(maxdm(ab), maxdm(cd),
+//                                   1/4 max(maxdm(ac), maxdm(ad), maxdm(bc), maxdm(bd)))
+// (J only: max(maxdm(ab), maxdm(cd)), fock_build_non_diag_j.f90:61-62);
+// survivors have bound >= thr.  Same-type classes take unordered pairs of
+// distinct pairs (ket position < bra position, fock_build_non_diag.f90:52-57).
+// With mixed precision, survivors with bound < split go to the FP32 list.
+__global__ void k_screen(const ScreenArgs s) {
+  const long long n = (long long)s.nbra * s.nket;
+  // Every lane of a wave must reach wave_append, so the loop bound is
+  // rounded up to whole waves.
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long nround = ((n + 63) / 64) * 64;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < nround; idx += stride) {
+    bool ok = idx < n;
+    int i = 0, j = 0;
+    if (ok) {
+      i = (int)(idx / s.nket);
+      j = (int)(idx - (long long)i * s.nket);
+      if (s.same && j >= i) ok = false;
+    }
+    bool keep64 = false, keep32 = false;
+    int gb = 0, gk = 0;
+    unsigned long long primq = 0, brap = 0;
+    if (ok) {
+      gb = s.bra[i];
+      gk = s.ket[j];
+      const PairRec b = s.pairs[gb];
+      const PairRec k = s.pairs[gk];
+      const int ns = s.nsh;
+      const double* md = s.maxdm;
+      double mv = fmax(md[(size_t)b.sha * ns + b.shb], md[(size_t)k.sha * ns + k.shb]);
+      if (!s.jonly) {
+        const double ex = fmax(fmax(md[(size_t)b.sha * ns + k.sha], md[(size_t)b.sha * ns + k.shb]),
+                               fmax(md[(size_t)b.shb * ns + k.sha], md[(size_t)b.shb * ns + k.shb]));
+        mv = fmax(mv, 0.25 * ex);
+      }
+      const double bound = s.schw_bra[i] * s.schw_ket[j] * mv;
+      if (!(bound < s.thr)) {
+        if (s.split > 0.0 && bound < s.split) keep32 = true; else keep64 = true;
+        const int nb_ = s.all_prims ? b.npall : b.np;
+        const int nk_ = s.all_prims ? k.npall : k.np;
+        primq = (unsigned long long)nb_ * (unsigned long long)nk_;
+        brap = (unsigned long long)nb_;
+      }
+    }
+    const int2 v = make_int2(gb, gk);
+    wave_append(keep64, v, s.list64, s.count64, s.stats64, primq, brap);
+    if (s.split > 0.0) wave_append(keep32, v, s.list32, s.count32, s.stats32, primq, brap);
+  }
+}
+
+// Diagonal-pair screening (fock_build_diag.f90:17-21): every pair of the type
+// (not truncated by `last`), bound = schw^2 * max(maxdm(ab), maxdm(bb)/4, maxdm(aa)/4).
+// With check == 0 every pair is taken (fock_build_initial_diag.f90:31-35).
+__global__ void k_screen_diag(const DiagScreenArgs s) {
+  const int nround = ((s.npair + 63) / 64) * 64;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nround; i += gridDim.x * blockDim.x) {
+    bool ok = i < s.npair;
+    int g = 0;
+    unsigned long long primq = 0, brap = 0;
+    if (ok) {
+      g = s.first + i;
+      const PairRec b = s.pairs[g];
+      if (s.check) {
+        const int ns = s.nsh;
+        const double mv = fmax(s.maxdm[(size_t)b.sha * ns + b.shb],
+                               fmax(0.25 * s.maxdm[(size_t)b.shb * ns + b.shb],
+                                    0.25 * s.maxdm[(size_t)b.sha * ns + b.sha]));
+        const double sc = s.schw[i];
+        if (sc * sc * mv < s.thr) ok = false;
+      }
+      if (ok) {
+        const int np_ = s.all_prims ? b.npall : b.np;
+        primq = (unsigned long long)np_ * np_;
+        brap = np_;
+      }
+    }
+    wave_append(ok, make_int2(g, g), s.list, s.count, s.stats, primq, brap);
+  }
+}
+
+// F += X + X^T (the accumulator holds each symmetric contribution once).
+__global__ void k_add_sym(const double* __restrict__ X, double* __restrict__ F, int nb) {
+  const long long n = (long long)nb * nb;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / nb), j = (int)(idx - (long long)i * nb);
+    F[idx] += X[idx] + X[(size_t)j * nb + i];
+  }
+}
+
+}  // namespace hfx

@@ -1,0 +1,603 @@
+"""Generator of the per-class gfx950 ERI + Fock-digestion kernels.
+
+One HIP translation unit per shell-quartet class (ab|cd), a, b, c, d in
+{s, p, d, l}.  The 46 classes and their order are those of the class loops of
+fock_build_non_diag.f90 (:52 (ss|ss) ... :3289 (ll|ll)); the integrals are
+the ones the reference's i2e_* routines return (s_sp_l_terms.f90,
+s_p_d_terms.f90, s_d_l_terms.f90).  The algorithm is written from the
+Head-Gordon--Pople / Obara--Saika equations, not from those routines:
+
+1. primitive loop over bra x ket primitive pairs:
+   [00|00]^(m) = K_ab K_cd (p+q)^-1/2 F_m(T), Boys F_m from an LDS table;
+   vertical recursion on A, then on C (HGP eqs. 6 and its ket twin) up to
+   [e0|f0]^(0), e in [la, la+lb], f in [lc, lc+ld];
+2. contraction, with the L-shell p/s coefficient ratio (djkl,
+   spread_basis.f90:149-151) applied to the p parts of L shells;
+3. horizontal recursion (a,b+1_i| = (a+1_i,b| + AB_i (a,b| on bra then ket;
+4. Cartesian -> spherical d (z2, xz, yz, xy, x2-y2; index_functions.f90:147-168);
+5. streamed digestion: every element updates six partial J/K contractions in
+   FP64; the quartet is dropped if max|e| < thr (fock_build_non_diag.f90:69)
+   and otherwise added to the accumulator X with atomics (F += X + X^T).
+
+Within a shell pair the shell of higher angular momentum is the build centre
+(no HRR for (ds|, (ls|, ...); the digestion works on AO indices, so the
+orientation does not change any result.
+
+Run as ``python -m hfcode_amd.gen.gen_eri OUTDIR`` (the build does this).
+It also writes ``flops_per_class.json``: operation counts of the emitted
+code (add = mul = 1, fma = 2), per primitive quartet, per bra primitive and
+per contracted quartet, which price the algorithmic FLOPs of a build.
+"""
+import json
+import os
+import sys
+
+CLASSES = ("ssss sssp sssd sssl sspp sspd sspl ssdd ssdl ssll spsp spsd spsl sppp sppd "
+           "sppl spdd spll sdsd sdsl sdpp sdpd sddd sddl sdll slsl slpp slpl sldd sldl "
+           "slll pppp pppd pppl ppdd ppll pdpd pddd plpl plll dddd dddl ddll dldl dlll llll").split()
+PAIR_TYPES = ["ss", "sp", "sd", "sl", "pp", "pd", "pl", "dd", "dl", "ll"]
+
+# angular parts of a shell: (l, weighted-by-djkl)
+PARTS = {"s": [(0, False)], "p": [(1, False)], "d": [(2, False)], "l": [(0, False), (1, True)]}
+NCOMP = {"s": 1, "p": 3, "d": 5, "l": 4}
+ORIENT_RANK = {"s": 0, "l": 1, "p": 2, "d": 3}
+AX = "xyz"
+SQ3 = "1.7320508075688772"
+HSQ3 = "0.8660254037844386"
+BOYS_TERMS = {"double": 8, "float": 5}
+BOYS_ROWS = 449          # grid 0, 1/16, ..., 28
+
+
+def cart(l):
+    """Cartesian exponent triples of angular momentum l, in the reference's
+    order (index_functions.f90:57-68): x^l first, z^l last."""
+    out = []
+    for i in range(l + 1):
+        lx = l - i
+        for n in range(i + 1):
+            out.append((lx, i - n, n))
+    return out
+
+
+def tname(t):
+    return "%d%d%d" % t
+
+
+def sub1(t, i):
+    return tuple(v - (1 if k == i else 0) for k, v in enumerate(t))
+
+
+def add1(t, i):
+    return tuple(v + (1 if k == i else 0) for k, v in enumerate(t))
+
+
+def comps(sh):
+    """Per AO component of shell type `sh`: (part index, kind, data).
+    kind 'c' -> Cartesian triple, kind 'sph' -> spherical d index 0..4."""
+    if sh == "s":
+        return [(0, "c", (0, 0, 0))]
+    if sh == "p":
+        return [(0, "c", t) for t in cart(1)]
+    if sh == "d":
+        return [(0, "sph", k) for k in range(5)]
+    return [(0, "c", (0, 0, 0))] + [(1, "c", t) for t in cart(1)]
+
+
+# spherical d from Cartesian d (order xx, xy, xz, yy, yz, zz), matharmond
+# (index_functions.f90:151-166)
+SPH = [
+    [((0, 0, 2), "1"), ((2, 0, 0), "-0.5"), ((0, 2, 0), "-0.5")],
+    [((1, 0, 1), SQ3)],
+    [((0, 1, 1), SQ3)],
+    [((1, 1, 0), SQ3)],
+    [((2, 0, 0), HSQ3), ((0, 2, 0), "-" + HSQ3)],
+]
+
+
+class Ops:
+    def __init__(self):
+        self.n = 0
+
+
+class Gen:
+    def __init__(self, cls):
+        self.cls = cls
+        ta, tb, tc, td = cls
+        self.bra_type = ta + tb
+        self.ket_type = tc + td
+        self.swap_bra = ORIENT_RANK[tb] > ORIENT_RANK[ta]
+        self.swap_ket = ORIENT_RANK[td] > ORIENT_RANK[tc]
+        self.A, self.B = (tb, ta) if self.swap_bra else (ta, tb)
+        self.C, self.D = (td, tc) if self.swap_ket else (tc, td)
+        self.bra_parts = [(pa, pb) for pa in range(len(PARTS[self.A])) for pb in range(len(PARTS[self.B]))]
+        self.ket_parts = [(pc, pd) for pc in range(len(PARTS[self.C])) for pd in range(len(PARTS[self.D]))]
+        self.has_L_bra = "l" in (self.A, self.B)
+        self.has_L_ket = "l" in (self.C, self.D)
+        la_max = max(l for l, _ in PARTS[self.A]) + max(l for l, _ in PARTS[self.B])
+        lc_max = max(l for l, _ in PARTS[self.C]) + max(l for l, _ in PARTS[self.D])
+        self.M = la_max + lc_max
+        # (sl|sl) is the one class routine that does not skip negligible
+        # primitive pairs (i2e_slsl, s_d_l_terms.f90:1629-1693).
+        self.all_prims = (cls == "slsl")
+
+    # ---- ranges -----------------------------------------------------------
+    def erange(self, bp):
+        la = PARTS[self.A][bp[0]][0]
+        lb = PARTS[self.B][bp[1]][0]
+        return la, lb, list(range(la, la + lb + 1))
+
+    def frange(self, kp):
+        lc = PARTS[self.C][kp[0]][0]
+        ld = PARTS[self.D][kp[1]][0]
+        return lc, ld, list(range(lc, lc + ld + 1))
+
+    # ---- primitive-level VRR ---------------------------------------------
+    def gen_vrr(self, lines, ops, targets):
+        memo = {}
+
+        def get(e, f, m):
+            if min(e) < 0 or min(f) < 0:
+                return None
+            key = (e, f, m)
+            if key in memo:
+                return memo[key]
+            if sum(e) == 0 and sum(f) == 0:
+                memo[key] = "s%d" % m
+                return memo[key]
+            name = "v%s_%s_%d" % (tname(e), tname(f), m)
+            if sum(f) == 0:
+                i = next(k for k in range(3) if e[k] > 0)
+                e1 = sub1(e, i)
+                t1 = get(e1, f, m); t2 = get(e1, f, m + 1)
+                expr = "PA%s * %s + WP%s * %s" % (AX[i], t1, AX[i], t2)
+                ops.n += 3
+                k = e[i] - 1
+                if k > 0:
+                    e2 = sub1(e1, i)
+                    t3 = get(e2, f, m); t4 = get(e2, f, m + 1)
+                    expr += " + (R)%d * oo2p * (%s - rho_p * %s)" % (k, t3, t4)
+                    ops.n += 4
+            else:
+                i = next(k for k in range(3) if f[k] > 0)
+                f1 = sub1(f, i)
+                t1 = get(e, f1, m); t2 = get(e, f1, m + 1)
+                expr = "QC%s * %s + WQ%s * %s" % (AX[i], t1, AX[i], t2)
+                ops.n += 3
+                k = f[i] - 1
+                if k > 0:
+                    f2 = sub1(f1, i)
+                    t3 = get(e, f2, m); t4 = get(e, f2, m + 1)
+                    expr += " + (R)%d * oo2q * (%s - rho_q * %s)" % (k, t3, t4)
+                    ops.n += 4
+                if e[i] > 0:
+                    t5 = get(sub1(e, i), f1, m + 1)
+                    expr += " + (R)%d * oo2pq * %s" % (e[i], t5)
+                    ops.n += 2
+            lines.append("const R %s = %s;" % (name, expr))
+            memo[key] = name
+            return name
+
+        out = {}
+        for (e, f) in targets:
+            out[(e, f)] = get(e, f, 0)
+        return out
+
+    # ---- code emission ----------------------------------------------------
+    def kernel(self):
+        L = []
+        W = L.append
+        cls = self.cls
+        ops_prim, ops_bra, ops_ctr = Ops(), Ops(), Ops()
+        M = self.M
+
+        # union of e (bra) and f (ket) Cartesian targets
+        e_all = sorted({l for bp in self.bra_parts for l in self.erange(bp)[2]})
+        f_by_kp = {kp: self.frange(kp)[2] for kp in self.ket_parts}
+        e_by_bp = {bp: self.erange(bp)[2] for bp in self.bra_parts}
+        targets = []
+        for kp in self.ket_parts:
+            for le in e_all:
+                for lf in f_by_kp[kp]:
+                    for e in cart(le):
+                        for f in cart(lf):
+                            if (e, f) not in targets:
+                                targets.append((e, f))
+        two_level = self.has_L_bra
+
+        def wket(kp):
+            terms = []
+            if PARTS[self.C][kp[0]][1]:
+                terms.append("wC")
+            if PARTS[self.D][kp[1]][1]:
+                terms.append("wD")
+            return "*".join(terms)
+
+        def wbra(bp):
+            terms = []
+            if PARTS[self.A][bp[0]][1]:
+                terms.append("wA")
+            if PARTS[self.B][bp[1]][1]:
+                terms.append("wB")
+            return "*".join(terms)
+
+        def acc_name(bp, kp, e, f):
+            return "acc_%d%d_%d%d_%s_%s" % (bp[0], bp[1], kp[0], kp[1], tname(e), tname(f))
+
+        def tmp_name(kp, e, f):
+            return "tmp_%d%d_%s_%s" % (kp[0], kp[1], tname(e), tname(f))
+
+        # accumulators
+        acc_list = []
+        for bp in self.bra_parts:
+            for kp in self.ket_parts:
+                for le in e_by_bp[bp]:
+                    for lf in f_by_kp[kp]:
+                        for e in cart(le):
+                            for f in cart(lf):
+                                acc_list.append(acc_name(bp, kp, e, f))
+        tmp_list = []
+        if two_level:
+            for kp in self.ket_parts:
+                for le in e_all:
+                    for lf in f_by_kp[kp]:
+                        for e in cart(le):
+                            for f in cart(lf):
+                                tmp_list.append(tmp_name(kp, e, f))
+
+        oa = "b" if self.swap_bra else "a"    # PairRec field suffix for oriented A
+        ob = "a" if self.swap_bra else "b"
+        oc = "b" if self.swap_ket else "a"
+        od = "a" if self.swap_ket else "b"
+        wa_f = "wb" if self.swap_bra else "wa"
+        wb_f = "wa" if self.swap_bra else "wb"
+        wc_f = "wb" if self.swap_ket else "wa"
+        wd_f = "wa" if self.swap_ket else "wb"
+
+        W("// (%s|%s) generated by hfcode_amd/gen/gen_eri.py -- do not edit" % (cls[:2], cls[2:]))
+        W("// build centres: A=%s B=%s C=%s D=%s; Boys m <= %d" % (self.A, self.B, self.C, self.D, M))
+        W("#include \"hfx_device.h\"")
+        W("namespace hfx {")
+        W("template <typename R, bool JONLY>")
+        W("__global__ void __launch_bounds__(HFX_NT) eri_%s(const ClassArgs a) {" % cls)
+        W("  extern __shared__ char hfx_smem[];")
+        W("  R* btab = reinterpret_cast<R*>(hfx_smem);")
+        W("  constexpr int NB_COL = %d + BoysTerms<R>::K;" % M)
+        W("  stage_boys<R>(btab, a, NB_COL);")
+        W("  const int nq = *a.count;")
+        W("  const int nbf = a.nb;")
+        W("  for (int iq = blockIdx.x * blockDim.x + threadIdx.x; iq < nq; iq += gridDim.x * blockDim.x) {")
+        W("    const int2 bk = a.list[iq];")
+        W("    const PairRec& pb = a.pairs[bk.x];")
+        W("    const PairRec& pk = a.pairs[bk.y];")
+        W("    const double Ax = pb.%sx, Ay = pb.%sy, Az = pb.%sz;" % (oa, oa, oa))
+        W("    const double Cx = pk.%sx, Cy = pk.%sy, Cz = pk.%sz;" % (oc, oc, oc))
+        W("    const R ABx = (R)(Ax - pb.%sx), ABy = (R)(Ay - pb.%sy), ABz = (R)(Az - pb.%sz);" % (ob, ob, ob))
+        W("    const R CDx = (R)(Cx - pk.%sx), CDy = (R)(Cy - pk.%sy), CDz = (R)(Cz - pk.%sz);" % (od, od, od))
+        W("    const int shA = pb.sh%s, shB = pb.sh%s, shC = pk.sh%s, shD = pk.sh%s;" % (oa, ob, oc, od))
+        W("    const int aoA = pb.ao%s, aoB = pb.ao%s, aoC = pk.ao%s, aoD = pk.ao%s;" % (oa, ob, oc, od))
+        for n in acc_list:
+            W("    R %s = 0;" % n)
+        npb = "pb.npall" if self.all_prims else "pb.np"
+        npk = "pk.npall" if self.all_prims else "pk.np"
+        W("    for (int ib = 0; ib < %s; ++ib) {" % npb)
+        W("      const PrimRec& rb = a.prims[pb.p0 + ib];")
+        W("      const R p = (R)rb.p, pinv = (R)rb.pinv, Kab = (R)rb.K;")
+        W("      const double Pxd = rb.Px, Pyd = rb.Py, Pzd = rb.Pz;")
+        W("      const R PAx = (R)(Pxd - Ax), PAy = (R)(Pyd - Ay), PAz = (R)(Pzd - Az);")
+        if self.has_L_bra:
+            W("      const R wA = (R)rb.%s, wB = (R)rb.%s; (void)wA; (void)wB;" % (wa_f, wb_f))
+        W("      const R oo2p = (R)0.5 * pinv; (void)oo2p;")
+        ops_bra.n += 4
+        for n in tmp_list:
+            W("      R %s = 0;" % n)
+        W("      for (int ik = 0; ik < %s; ++ik) {" % npk)
+        W("        const PrimRec& rk = a.prims[pk.p0 + ik];")
+        W("        const R q = (R)rk.p, qinv = (R)rk.pinv, Kcd = (R)rk.K;")
+        W("        const R QCx = (R)(rk.Px - Cx), QCy = (R)(rk.Py - Cy), QCz = (R)(rk.Pz - Cz);")
+        W("        const R PQx = (R)(Pxd - rk.Px), PQy = (R)(Pyd - rk.Py), PQz = (R)(Pzd - rk.Pz);")
+        if self.has_L_ket:
+            W("        const R wC = (R)rk.%s, wD = (R)rk.%s; (void)wC; (void)wD;" % (wc_f, wd_f))
+        W("        const R ipq = (R)1 / (p + q);")
+        W("        const R rho_p = q * ipq, rho_q = p * ipq; (void)rho_p; (void)rho_q;")
+        W("        const R T = p * rho_p * (PQx * PQx + PQy * PQy + PQz * PQz);")
+        W("        const R oo2q = (R)0.5 * qinv, oo2pq = (R)0.5 * ipq; (void)oo2q; (void)oo2pq;")
+        W("        const R WPx = -rho_p * PQx, WPy = -rho_p * PQy, WPz = -rho_p * PQz;")
+        W("        const R WQx = rho_q * PQx, WQy = rho_q * PQy, WQz = rho_q * PQz;")
+        W("        (void)WPx; (void)WPy; (void)WPz; (void)WQx; (void)WQy; (void)WQz;")
+        W("        (void)QCx; (void)QCy; (void)QCz; (void)PAx; (void)PAy; (void)PAz;")
+        # ops: 1 add + div(1) + 2 mul + T(1 mul + 5 for dot) + 2 mul + 6 mul + 9 sub(PQ,QC)
+        ops_prim.n += 1 + 1 + 2 + 6 + 2 + 6 + 6
+        W("        R F[%d];" % (M + 1))
+        W("        boys<R, %d>(btab, NB_COL, T, F);" % M)
+        W("        const R pref = Kab * Kcd * sqrt(ipq);")
+        ops_prim.n += 3
+        for m in range(M + 1):
+            W("        const R s%d = pref * F[%d];" % (m, m))
+        ops_prim.n += M + 1
+        vlines = []
+        vops = Ops()
+        vmap = self.gen_vrr(vlines, vops, targets)
+        ops_prim.n += vops.n
+        for ln in vlines:
+            W("        " + ln)
+        # contraction
+        for kp in self.ket_parts:
+            wk = wket(kp)
+            if two_level:
+                for le in e_all:
+                    for lf in f_by_kp[kp]:
+                        for e in cart(le):
+                            for f in cart(lf):
+                                v = vmap[(e, f)]
+                                if wk:
+                                    W("        %s += %s * %s;" % (tmp_name(kp, e, f), wk, v)); ops_prim.n += 2
+                                else:
+                                    W("        %s += %s;" % (tmp_name(kp, e, f), v)); ops_prim.n += 1
+            else:
+                bp = self.bra_parts[0]
+                for le in e_by_bp[bp]:
+                    for lf in f_by_kp[kp]:
+                        for e in cart(le):
+                            for f in cart(lf):
+                                v = vmap[(e, f)]
+                                if wk:
+                                    W("        %s += %s * %s;" % (acc_name(bp, kp, e, f), wk, v)); ops_prim.n += 2
+                                else:
+                                    W("        %s += %s;" % (acc_name(bp, kp, e, f), v)); ops_prim.n += 1
+        W("      }")
+        if two_level:
+            for bp in self.bra_parts:
+                wb = wbra(bp)
+                for kp in self.ket_parts:
+                    for le in e_by_bp[bp]:
+                        for lf in f_by_kp[kp]:
+                            for e in cart(le):
+                                for f in cart(lf):
+                                    t = tmp_name(kp, e, f)
+                                    if wb:
+                                        W("      %s += %s * %s;" % (acc_name(bp, kp, e, f), wb, t)); ops_bra.n += 2
+                                    else:
+                                        W("      %s += %s;" % (acc_name(bp, kp, e, f), t)); ops_bra.n += 1
+        W("    }")
+        # ---- HRR + spherical per part combination -> element expressions ----
+        elem = {}     # (ia, ib, ic, id) -> variable name
+        cA, cB, cC, cD = comps(self.A), comps(self.B), comps(self.C), comps(self.D)
+        for bp in self.bra_parts:
+            la, lb, _ = self.erange(bp)
+            for kp in self.ket_parts:
+                lc, ld, fr = self.frange(kp)
+                tag = "%d%d%d%d" % (bp + kp)
+                fl = [f for lf in fr for f in cart(lf)]
+                # bra HRR
+                hm = {}
+
+                def hb(a_, b_, f_):
+                    key = (a_, b_, f_)
+                    if key in hm:
+                        return hm[key]
+                    if sum(b_) == 0:
+                        hm[key] = acc_name(bp, kp, a_, f_)
+                        return hm[key]
+                    i = next(k for k in range(3) if b_[k] > 0)
+                    x1 = hb(add1(a_, i), sub1(b_, i), f_)
+                    x2 = hb(a_, sub1(b_, i), f_)
+                    nm = "hb%s_%s_%s_%s" % (tag, tname(a_), tname(b_), tname(f_))
+                    W("    const R %s = %s + AB%s * %s;" % (nm, x1, AX[i], x2))
+                    ops_ctr.n += 2
+                    hm[key] = nm
+                    return nm
+
+                # bra components (after spherical) as functions of f
+                def bra_val(ca, cb, f_):
+                    # returns expression name for (ca cb| f_) with spherical handling
+                    def cart_terms(c, l):
+                        if c[1] == "c":
+                            return [(c[2], "1")]
+                        return SPH[c[2]]
+                    terms = []
+                    for (ta_, fa) in cart_terms(ca, la):
+                        for (tb_, fb) in cart_terms(cb, lb):
+                            terms.append((hb(ta_, tb_, f_), fa, fb))
+                    return terms
+
+                braname = {}
+                for ia, ca in enumerate(cA):
+                    if ca[0] != bp[0]:
+                        continue
+                    for ib, cb in enumerate(cB):
+                        if cb[0] != bp[1]:
+                            continue
+                        for f_ in fl:
+                            terms = bra_val(ca, cb, f_)
+                            if len(terms) == 1 and terms[0][1] == "1" and terms[0][2] == "1":
+                                braname[(ia, ib, f_)] = terms[0][0]
+                            else:
+                                nm = "bs%s_%d_%d_%s" % (tag, ia, ib, tname(f_))
+                                ex = " + ".join("(R)(%s*%s) * %s" % (fa, fb, t) if (fa, fb) != ("1", "1") else t
+                                                for (t, fa, fb) in terms)
+                                W("    const R %s = %s;" % (nm, ex))
+                                ops_ctr.n += 2 * len(terms) - 1
+                                braname[(ia, ib, f_)] = nm
+                # ket HRR for each bra component
+                for ia, ca in enumerate(cA):
+                    if ca[0] != bp[0]:
+                        continue
+                    for ib, cb in enumerate(cB):
+                        if cb[0] != bp[1]:
+                            continue
+                        km = {}
+
+                        def hk(c_, d_):
+                            key = (c_, d_)
+                            if key in km:
+                                return km[key]
+                            if sum(d_) == 0:
+                                km[key] = braname[(ia, ib, c_)]
+                                return km[key]
+                            i = next(k for k in range(3) if d_[k] > 0)
+                            x1 = hk(add1(c_, i), sub1(d_, i))
+                            x2 = hk(c_, sub1(d_, i))
+                            nm = "hk%s_%d_%d_%s_%s" % (tag, ia, ib, tname(c_), tname(d_))
+                            W("    const R %s = %s + CD%s * %s;" % (nm, x1, AX[i], x2))
+                            ops_ctr.n += 2
+                            km[key] = nm
+                            return nm
+
+                        for ic, cc in enumerate(cC):
+                            if cc[0] != kp[0]:
+                                continue
+                            for id_, cd in enumerate(cD):
+                                if cd[0] != kp[1]:
+                                    continue
+                                tc_ = [(cc[2], "1")] if cc[1] == "c" else SPH[cc[2]]
+                                td_ = [(cd[2], "1")] if cd[1] == "c" else SPH[cd[2]]
+                                terms = [(hk(x, y), fx, fy) for (x, fx) in tc_ for (y, fy) in td_]
+                                nm = "e_%d_%d_%d_%d" % (ia, ib, ic, id_)
+                                if len(terms) == 1 and terms[0][1] == "1" and terms[0][2] == "1":
+                                    W("    const R %s = %s;" % (nm, terms[0][0]))
+                                else:
+                                    ex = " + ".join("(R)(%s*%s) * %s" % (fx, fy, t) if (fx, fy) != ("1", "1") else t
+                                                    for (t, fx, fy) in terms)
+                                    W("    const R %s = %s;" % (nm, ex))
+                                    ops_ctr.n += 2 * len(terms) - 1
+                                elem[(ia, ib, ic, id_)] = nm
+        nA, nB, nC, nD = len(cA), len(cB), len(cC), len(cD)
+        assert len(elem) == nA * nB * nC * nD
+        # ---- digestion -------------------------------------------------------
+        W("    // density blocks")
+
+        def pblk(name, ao1, n1, ao2, n2, konly):
+            for i in range(n1):
+                for j in range(n2):
+                    ld = "a.P[(size_t)(%s + %d) * nbf + %s + %d]" % (ao1, i, ao2, j)
+                    if konly:
+                        ld = "JONLY ? 0.0 : " + ld
+                    W("    const double %s_%d_%d = %s;" % (name, i, j, ld))
+
+        pblk("Pab", "aoA", nA, "aoB", nB, False)
+        pblk("Pcd", "aoC", nC, "aoD", nD, False)
+        pblk("Pac", "aoA", nA, "aoC", nC, True)
+        pblk("Pad", "aoA", nA, "aoD", nD, True)
+        pblk("Pbc", "aoB", nB, "aoC", nC, True)
+        pblk("Pbd", "aoB", nB, "aoD", nD, True)
+
+        def decl(name, n1, n2):
+            for i in range(n1):
+                for j in range(n2):
+                    W("    double %s_%d_%d = 0;" % (name, i, j))
+
+        decl("Jab", nA, nB); decl("Jcd", nC, nD)
+        decl("Kac", nA, nC); decl("Kad", nA, nD); decl("Kbc", nB, nC); decl("Kbd", nB, nD)
+        W("    R vmax = 0;")
+        W("    const R ecut = a.diag ? (R)HFX_INTTHRESH : (R)0;")
+        W("    const bool dump = a.blocks != nullptr;")
+        # reference-orientation linear index for the debug block dump
+        n_ref = [NCOMP[x] for x in cls]
+        for (ia, ib, ic, id_), nm in sorted(elem.items()):
+            W("    {")
+            W("      R v = %s;" % nm)
+            W("      vmax = fmax(vmax, fabs(v));")
+            W("      if (fabs(v) <= ecut) v = 0;")
+            i1, i2 = (ib, ia) if self.swap_bra else (ia, ib)
+            i3, i4 = (id_, ic) if self.swap_ket else (ic, id_)
+            lin = i1 + n_ref[0] * (i2 + n_ref[1] * (i3 + n_ref[2] * i4))
+            W("      if (dump) a.blocks[(size_t)iq * 625 + %d] = (double)v;" % lin)
+            W("      const double dv = (double)v;")
+            W("      Jab_%d_%d = fma(dv, Pcd_%d_%d, Jab_%d_%d);" % (ia, ib, ic, id_, ia, ib))
+            W("      Jcd_%d_%d = fma(dv, Pab_%d_%d, Jcd_%d_%d);" % (ic, id_, ia, ib, ic, id_))
+            W("      if constexpr (!JONLY) {")
+            W("        Kac_%d_%d = fma(dv, Pbd_%d_%d, Kac_%d_%d);" % (ia, ic, ib, id_, ia, ic))
+            W("        Kad_%d_%d = fma(dv, Pbc_%d_%d, Kad_%d_%d);" % (ia, id_, ib, ic, ia, id_))
+            W("        Kbc_%d_%d = fma(dv, Pad_%d_%d, Kbc_%d_%d);" % (ib, ic, ia, id_, ib, ic))
+            W("        Kbd_%d_%d = fma(dv, Pac_%d_%d, Kbd_%d_%d);" % (ib, id_, ia, ic, ib, id_))
+            W("      }")
+            W("    }")
+            ops_ctr.n += 12 + 1
+        W("    if (a.schw_out) a.schw_out[bk.x] = sqrt((double)vmax);")
+        W("    if (!a.diag && (double)vmax < a.thr) continue;")
+        W("    const double w = ((shA == shB) ? 0.5 : 1.0) * ((shC == shD) ? 0.5 : 1.0) * (a.diag ? 0.5 : 1.0);")
+        W("    const double wj = 2.0 * w, wk = -0.5 * w;")
+        W("    double* X = a.X;")
+
+        def flush(name, ao1, n1, ao2, n2, wname):
+            for i in range(n1):
+                for j in range(n2):
+                    W("    atomicAdd(&X[(size_t)(%s + %d) * nbf + %s + %d], %s * %s_%d_%d);" % (ao1, i, ao2, j, wname, name, i, j))
+
+        flush("Jab", "aoA", nA, "aoB", nB, "wj")
+        flush("Jcd", "aoC", nC, "aoD", nD, "wj")
+        W("    if constexpr (!JONLY) {")
+        flush("Kac", "aoA", nA, "aoC", nC, "wk")
+        flush("Kad", "aoA", nA, "aoD", nD, "wk")
+        flush("Kbc", "aoB", nB, "aoC", nC, "wk")
+        flush("Kbd", "aoB", nB, "aoD", nD, "wk")
+        W("    }")
+        W("  }")
+        W("}")
+        W("template __global__ void eri_%s<double, false>(const ClassArgs);" % cls)
+        W("template __global__ void eri_%s<double, true>(const ClassArgs);" % cls)
+        W("template __global__ void eri_%s<float, false>(const ClassArgs);" % cls)
+        W("}  // namespace hfx")
+        info = {
+            "class": cls, "M": M, "bra": self.bra_type, "ket": self.ket_type,
+            "flops_prim_vrr": ops_prim.n,
+            "flops_braprim": ops_bra.n,
+            "flops_quartet": ops_ctr.n,
+            "all_prims": self.all_prims,
+        }
+        return "\n".join(L) + "\n", info
+
+
+def boys_flops(M, K):
+    """Table-Taylor Boys: K-1 products for the Taylor factors, then K-1 fma + 1
+    mul per order; plus index/offset arithmetic (3)."""
+    return (K - 1) + (M + 1) * (2 * (K - 1) + 1) + 3
+
+
+def main(outdir):
+    os.makedirs(outdir, exist_ok=True)
+    table = {}
+    names = []
+    for cls in CLASSES:
+        src, info = Gen(cls).kernel()
+        path = os.path.join(outdir, "eri_%s.hip" % cls)
+        old = open(path).read() if os.path.exists(path) else None
+        if old != src:
+            with open(path, "w") as fh:
+                fh.write(src)
+        info["flops_prim"] = info["flops_prim_vrr"] + boys_flops(info["M"], BOYS_TERMS["double"])
+        info["flops_prim_fp32"] = info["flops_prim_vrr"] + boys_flops(info["M"], BOYS_TERMS["float"])
+        table[cls] = info
+        names.append(cls)
+    # dispatch table
+    D = ["// generated by hfcode_amd/gen/gen_eri.py -- do not edit",
+         "#pragma once", "#include \"hfx_device.h\"", "namespace hfx {"]
+    for c in names:
+        D.append("template <typename R, bool JONLY> __global__ void eri_%s(const ClassArgs);" % c)
+    D.append("struct ClassKernels {")
+    D.append("  const void* f64; const void* f64j; const void* f32; int M; int all_prims;")
+    D.append("  double flops_prim, flops_prim_fp32, flops_braprim, flops_quartet;")
+    D.append("};")
+    D.append("static const ClassKernels kClassKernels[%d] = {" % len(names))
+    for c in names:
+        t = table[c]
+        t["flops_prim"] = t["flops_prim_vrr"] + boys_flops(t["M"], BOYS_TERMS["double"])
+        t["flops_prim_fp32"] = t["flops_prim_vrr"] + boys_flops(t["M"], BOYS_TERMS["float"])
+        D.append("  {(const void*)&eri_%s<double, false>, (const void*)&eri_%s<double, true>, "
+                 "(const void*)&eri_%s<float, false>, %d, %d, %d, %d, %d, %d}," % (
+                     c, c, c, t["M"], int(t["all_prims"]), t["flops_prim"], t["flops_prim_fp32"],
+                     t["flops_braprim"], t["flops_quartet"]))
+    D.append("};")
+    D.append("}  // namespace hfx")
+    path = os.path.join(outdir, "eri_dispatch.h")
+    src = "\n".join(D) + "\n"
+    old = open(path).read() if os.path.exists(path) else None
+    if old != src:
+        with open(path, "w") as fh:
+            fh.write(src)
+    with open(os.path.join(outdir, "flops_per_class.json"), "w") as fh:
+        json.dump(table, fh, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(os.path.dirname(__file__), "..", "csrc", "gen"))

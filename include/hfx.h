@@ -1,0 +1,125 @@
+/* hfx -- MI355X (gfx950) direct-SCF Fock build, C ABI.
+ *
+ * Drop-in for the two-electron Fock build of brichjnh/hfcode:
+ *   fock_build_non_diag(f,p,thr)  fock_build_non_diag.f90:1-10  (J - K/2)
+ *   fock_build_non_diag_k(...)    fock_build_non_diag_k.f90     (same routine)
+ *   fock_build_non_diag_j_only    fock_build_non_diag_j.f90:1   (J only)
+ *   fock_build_diag(f,p,thr)      fock_build_diag.f90:1
+ *   fock_build_initial_diag       fock_build_initial_diag.f90:1 (Schwarz tables)
+ * All matrices are nb x nb FP64; P must be symmetric.  F is ACCUMULATED into
+ * (F += G), never overwritten, as the reference routines do.  Plain pointers
+ * and sizes only; every call returns 0 or a negative HFX_E_* code.
+ */
+#ifndef HFX_H
+#define HFX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HFX_OK 0
+#define HFX_E_ARG (-1)          /* bad argument (null pointer, nb mismatch, ...) */
+#define HFX_E_UNSUPPORTED (-2)  /* basis the reference rejects (P+D+L mix, > 10 primitives) */
+#define HFX_E_HIP (-3)          /* HIP runtime error (no device, launch failure) */
+#define HFX_E_NOMEM (-4)        /* device allocation failed */
+#define HFX_E_STATE (-5)        /* Schwarz tables missing: run hfx_fock_initial_diag first */
+
+/* build modes */
+#define HFX_JK 0                /* F += J - K/2        (fock_build_non_diag / _k) */
+#define HFX_J_ONLY 1            /* F += J, Coulomb-only screening (fock_build_non_diag_j) */
+
+/* options for hfx_set_option */
+#define HFX_OPT_MIXED_SPLIT 1   /* > 0: quartets with Schwarz x density bound below this value
+                                   are evaluated in FP32 (accumulation stays FP64); 0 = all FP64 */
+#define HFX_OPT_DEBUG_BLOCKS 2  /* 1: keep every integral block of the next build (tests) */
+
+/* pair types, in the order of the reference's pair lists (molprops.f90:122-131) */
+#define HFX_NPAIRTYPES 10       /* ss sp sd sl pp pd pl dd dl ll */
+#define HFX_NCLASSES 46         /* (ss|ss) ... (ll|ll), order of fock_build_non_diag.f90 */
+
+typedef struct hfx_ctx hfx_ctx;
+
+/* Molecule + basis, as held by module molprops (molprops.f90:37-168).
+ * Shell arrays are indexed by shell; all indices are 1-based, as in Fortran. */
+typedef struct hfx_mol_desc {
+  int nb;                      /* basis functions (5 spherical d, L = s+p) */
+  int nsh;                     /* shells */
+  int maxprim;                 /* leading dimension of zeta/djk/djkl (molprops maxprim = 10) */
+  const int* shtyp;            /* [nsh] 1 = s, 2 = p, 3 = d, 12 = L  (shtyp) */
+  const int* nzet;             /* [nsh] primitives per shell          (nzet) */
+  const int* aosh;             /* [nsh] first AO of the shell, 1-based (aosh) */
+  const double* xsh;           /* [3*nsh] centres in bohr, (3,nsh)    (xsh) */
+  const double* zeta;          /* [maxprim*nsh] exponents             (zeta) */
+  const double* djk;           /* [maxprim*nsh] normalised coefficients, s part for L (djk) */
+  const double* djkl;          /* [maxprim*nsh] L-shell p/s ratios; ignored for s, p, d (djkl) */
+  /* shell-pair lists and Schwarz tables per pair type (ssprs/schwss/orderss/lastss, ...).
+   * npair[t] = 0 skips the type.  schw/order may be NULL: the library then builds the
+   * Schwarz tables itself on the first hfx_fock_initial_diag call. */
+  int npair[HFX_NPAIRTYPES];
+  const int* prs[HFX_NPAIRTYPES];       /* [2*npair] shells of each pair, 1-based */
+  const double* schw[HFX_NPAIRTYPES];   /* [npair] sqrt(max|(ab|ab)|) */
+  const int* order[HFX_NPAIRTYPES];     /* [npair] descending Schwarz order, 1-based */
+  int last[HFX_NPAIRTYPES];             /* truncation index (index_sort.f90:95-119) */
+} hfx_mol_desc;
+
+/* Per-build statistics (of the most recent build on the context). */
+typedef struct hfx_stats {
+  long long quartets[HFX_NCLASSES][2];       /* surviving contracted quartets [fp64, fp32] */
+  long long prim_quartets[HFX_NCLASSES][2];  /* primitive quartets evaluated [fp64, fp32] */
+  long long bra_prims[HFX_NCLASSES][2];      /* bra primitive iterations [fp64, fp32] */
+  double flops_fp64, flops_fp32;             /* algorithmic FLOPs of the build */
+  double ms_total;                           /* device time of the build, ms */
+} hfx_stats;
+
+/* Create a context on the current HIP device: copies the shells, builds the
+ * primitive-pair data (precalc_kabs.f90:94-295) and the Boys table, and
+ * allocates the device buffers.  Returns HFX_E_UNSUPPORTED for bases the
+ * reference rejects. */
+int hfx_ctx_create(const hfx_mol_desc* desc, hfx_ctx** out);
+void hfx_ctx_destroy(hfx_ctx* ctx);
+int hfx_set_option(hfx_ctx* ctx, int key, double value);
+
+/* F += off-diagonal two-electron part of the Fock matrix.
+ * Replaces fock_build_non_diag(f,p,thr) (mode HFX_JK) and
+ * fock_build_non_diag_j_only(f,p,thr) (mode HFX_J_ONLY).
+ * P and F are host arrays of nb*nb doubles. */
+int hfx_fock_nd(hfx_ctx* ctx, const double* P, double* F, int nb, double thr, int mode);
+
+/* F += diagonal-pair part (ab|ab): replaces fock_build_diag(f,p,thr). */
+int hfx_fock_diag(hfx_ctx* ctx, const double* P, double* F, int nb, double thr);
+
+/* Builds the Schwarz tables from the diagonal quartets and adds their
+ * contribution to F: replaces fock_build_initial_diag(f,p,thr).  Uses the
+ * tables passed in hfx_mol_desc when present (they are then not rebuilt). */
+int hfx_fock_initial_diag(hfx_ctx* ctx, const double* P, double* F, int nb, double thr);
+
+/* Same builds on device-resident nb*nb arrays (hipMalloc'd), enqueued on the
+ * context's stream; `stream` (a hipStream_t, may be NULL for the context
+ * stream) is waited on before and made to wait after.  No host copies. */
+int hfx_fock_nd_device(hfx_ctx* ctx, const double* dP, double* dF, double thr, int mode, void* stream);
+int hfx_fock_diag_device(hfx_ctx* ctx, const double* dP, double* dF, double thr, void* stream);
+
+/* Statistics of the most recent build (blocks until it has finished). */
+int hfx_get_stats(hfx_ctx* ctx, hfx_stats* out);
+
+/* Schwarz tables as used by the context (after initial_diag when built by the
+ * library).  schw/order have npair[t] entries; order is 1-based. */
+int hfx_get_schwarz(hfx_ctx* ctx, int t, double* schw, int* order, int* last);
+
+/* Debug: integral blocks of the most recent build (HFX_OPT_DEBUG_BLOCKS = 1).
+ * For class c (0..45) and precision prec (0 fp64, 1 fp32): number of
+ * quartets; when quartets/blocks are non-NULL they receive (bra pair, ket
+ * pair) 0-based global pair indices and 625 doubles per quartet, each block
+ * laid out as the reference routine's output argument (column-major, s shells
+ * without a dimension). */
+long long hfx_debug_blocks(hfx_ctx* ctx, int c, int prec, int* quartets, double* blocks);
+
+/* Global pair index -> (type, 1-based position in the type's list). */
+int hfx_pair_info(hfx_ctx* ctx, int gpair, int* type, int* index);
+
+const char* hfx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,158 @@
+"""ORACLE-SIDE TEST INFRASTRUCTURE -- not part of the product.
+
+ctypes access to ``oracle/_ref/libhfref.so``: the reference Fortran
+(brichjnh/hfcode) compiled from its own sources by ``oracle/Makefile``, wrapped
+by ``oracle/ref_harness.f90``.  Only tests, the smoke check and the CPU
+baseline leg of bench.py use it, as the checker or as the timed CPU
+reference.  The reference keeps all molecule state in Fortran module
+variables, so one molecule is loaded per process at a time.
+"""
+import ctypes
+import os
+import tempfile
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_ref", "libhfref.so")
+
+PAIR_TYPES = ["ss", "sp", "sd", "sl", "pp", "pd", "pl", "dd", "dl", "ll"]
+# Class order of the loops in fock_build_non_diag.f90 (:52 ... :3289).
+CLASSES = ("ssss sssp sssd sssl sspp sspd sspl ssdd ssdl ssll spsp spsd spsl sppp sppd "
+           "sppl spdd spll sdsd sdsl sdpp sdpd sddd sddl sdll slsl slpp slpl sldd sldl "
+           "slll pppp pppd pppl ppdd ppll pdpd pddd plpl plll dddd dddl ddll dldl dlll llll").split()
+
+_lib = None
+
+
+def available():
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise FileNotFoundError("%s not built (make -C oracle)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.hfref_load.restype = ctypes.c_int
+        L.hfref_load.argtypes = [ctypes.c_char_p] * 3
+        L.hfref_dims.argtypes = [ip]
+        L.hfref_unuc.restype = ctypes.c_double
+        L.hfref_shells.argtypes = [ip, ip, ip, ip, dp, dp, dp, dp]
+        L.hfref_pairs.restype = ctypes.c_int
+        L.hfref_pairs.argtypes = [ctypes.c_int, ip, dp, ip, ip]
+        L.hfref_onee.argtypes = [dp, dp, dp]
+        for n in ("hfref_initial_diag", "hfref_diag", "hfref_non_diag"):
+            getattr(L, n).argtypes = [dp, dp, ctypes.c_double]
+        L.hfref_maxdm.argtypes = [dp, dp]
+        L.hfref_boys.argtypes = [ctypes.c_int, ctypes.c_double, dp]
+        L.hfref_eri.argtypes = [ctypes.c_int] * 5 + [dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def boys(n, x):
+    out = np.zeros(n + 1)
+    lib().hfref_boys(n, float(x), _dp(out))
+    return out
+
+
+class RefMolecule:
+    """One molecule loaded into the reference (set-up of hf.f90:24-33)."""
+
+    def __init__(self, inp_text, dat_text):
+        L = lib()
+        self._tmp = tempfile.TemporaryDirectory(prefix="hfref")
+        d = self._tmp.name
+        with open(os.path.join(d, "m.inp"), "w") as fh:
+            fh.write(inp_text)
+        with open(os.path.join(d, "b.dat"), "w") as fh:
+            fh.write(dat_text)
+        cwd = os.getcwd()
+        os.chdir(d)
+        try:
+            self.nb = L.hfref_load(b"m.inp", b"b.dat", b"m.log")
+        finally:
+            os.chdir(cwd)
+        dims = np.zeros(32, dtype=np.int32)
+        L.hfref_dims(_ip(dims))
+        self.natom, self.nb, self.nsh, self.nocc, self.nab = [int(x) for x in dims[:5]]
+        self.maxprim = int(dims[9])
+        self.npairs = [int(x) for x in dims[10:20]]
+        self.unuc = L.hfref_unuc()
+        self.schwarz_ready = False
+
+    @classmethod
+    def from_molecule(cls, mol, basis):
+        from hfcode_amd.molecule import basis_to_dat
+        return cls(mol.to_inp(), basis_to_dat(basis))
+
+    def free(self):
+        lib().hfref_free()
+        self._tmp.cleanup()
+
+    def shells(self):
+        n, m = self.nsh, self.maxprim
+        ityp = np.zeros(n, np.int32); inz = np.zeros(n, np.int32)
+        iat = np.zeros(n, np.int32); iao = np.zeros(n, np.int32)
+        xyz = np.zeros((n, 3)); zet = np.zeros((n, m)); cof = np.zeros((n, m)); cofl = np.zeros((n, m))
+        lib().hfref_shells(_ip(ityp), _ip(inz), _ip(iat), _ip(iao), _dp(xyz), _dp(zet), _dp(cof), _dp(cofl))
+        return dict(shtyp=ityp, nzet=inz, atsh=iat, aosh=iao, xsh=xyz, zeta=zet, djk=cof, djkl=cofl)
+
+    def pairs(self):
+        """Per pair type: (prs (n,2) 1-based shells, schw, order 1-based, last)."""
+        out = []
+        for t in range(10):
+            n = self.npairs[t]
+            prs = np.zeros((max(n, 1), 2), np.int32)
+            sch = np.zeros(max(n, 1)); ordr = np.zeros(max(n, 1), np.int32)
+            last = ctypes.c_int(0)
+            lib().hfref_pairs(t + 1, _ip(prs), _dp(sch), _ip(ordr), ctypes.byref(last))
+            out.append((prs[:n].copy(), sch[:n].copy(), ordr[:n].copy(), int(last.value)))
+        return out
+
+    def onee(self):
+        nb = self.nb
+        s = np.zeros((nb, nb), order="F"); t = np.zeros((nb, nb), order="F"); v = np.zeros((nb, nb), order="F")
+        lib().hfref_onee(_dp(s), _dp(t), _dp(v))
+        return s, t, v
+
+    def _fock(self, name, f, p, thr):
+        f = np.asfortranarray(f, dtype=np.float64).copy(order="F")
+        p = np.asfortranarray(p, dtype=np.float64)
+        getattr(lib(), name)(_dp(f), _dp(p), float(thr))
+        return f
+
+    def initial_diag(self, f, p, thr):
+        self.schwarz_ready = True
+        return self._fock("hfref_initial_diag", f, p, thr)
+
+    def diag(self, f, p, thr):
+        return self._fock("hfref_diag", f, p, thr)
+
+    def non_diag(self, f, p, thr):
+        assert self.schwarz_ready, "call initial_diag first (it builds the Schwarz tables)"
+        return self._fock("hfref_non_diag", f, p, thr)
+
+    def maxdm(self, p):
+        out = np.zeros(self.nab)
+        lib().hfref_maxdm(_dp(np.asfortranarray(p, dtype=np.float64)), _dp(out))
+        return out
+
+    def eri(self, cls, sh1, sh2, sh3, sh4):
+        """Reference block of class `cls` (name or index) for 1-based shells."""
+        ci = CLASSES.index(cls) if isinstance(cls, str) else int(cls)
+        buf = np.zeros(625)
+        lib().hfref_eri(ci, sh1, sh2, sh3, sh4, _dp(buf))
+        return buf
