@@ -4,6 +4,9 @@
 #include <stdint.h>
 
 #define HFX_NT 256                 // threads per workgroup of the class kernels
+#ifndef HFX_WAVE
+#define HFX_WAVE 64                // wavefront width of gfx950
+#endif
 #define HFX_INTTHRESH 1.0e-10      // element cut of the diagonal builds (molprops.f90:162)
 #define HFX_BOYS_ROWS 449          // Boys grid x = 0, 1/16, ..., 28
 #define HFX_BOYS_COLS 16           // F_0 .. F_15 per grid point
@@ -91,7 +94,8 @@ __device__ __forceinline__ void boys(const R* __restrict__ tab, int ncol, R T, R
 #pragma unroll
     for (int m = 1; m <= M; ++m) F[m] = F[m - 1] * ((R)m - (R)0.5) * t;
   } else {
-    const int i = (int)(T * (R)HFX_BOYS_STEP + (R)0.5);
+    int i = (int)(T * (R)HFX_BOYS_STEP + (R)0.5);
+    i = i < 0 ? 0 : (i > HFX_BOYS_ROWS - 1 ? HFX_BOYS_ROWS - 1 : i);
     const R d = (R)i * (R)(1.0 / HFX_BOYS_STEP) - T;
     R c[K];
     c[0] = 1;

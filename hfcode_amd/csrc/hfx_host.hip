@@ -215,16 +215,16 @@ int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
 
 int zero_build_state(hfx_ctx* c) {
   const size_t nx = (size_t)c->nb * c->nb;
-  k_fill_f64<<<grid_for((long long)nx, 256, 2048), 256, 0, c->stream>>>(c->d_X, nx, 0.0);
-  k_fill_i32<<<1, 256, 0, c->stream>>>(c->d_counts, 2 * HFX_NCLASSES, 0);
-  k_fill_u64<<<1, 256, 0, c->stream>>>(c->d_stats, 2 * HFX_NCLASSES * 3);
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)nx, 256, 2048)), dim3(256), 0, c->stream, c->d_X, nx, 0.0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, c->stream, c->d_counts, 2 * HFX_NCLASSES, 0);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, c->stream, c->d_stats, 2 * HFX_NCLASSES * 3);
   HIPOK(hipGetLastError());
   return HFX_OK;
 }
 
 int launch_maxdm(hfx_ctx* c, const double* dP) {
   const long long n = (long long)c->nsh * c->nsh;
-  k_maxdm<<<grid_for(n, 256, 4096), 256, 0, c->stream>>>(dP, c->nb, c->d_shells, c->nsh, c->d_maxdm);
+  hipLaunchKernelGGL(k_maxdm, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->stream, dP, c->nb, c->d_shells, c->nsh, c->d_maxdm);
   HIPOK(hipGetLastError());
   return HFX_OK;
 }
@@ -274,7 +274,7 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode) {
     s.nsh = c->nsh;
     s.all_prims = kClassKernels[cp.cls].all_prims;
     const long long n = (long long)s.nbra * s.nket;
-    k_screen<<<grid_for(n, 256, c->ncu * 16), 256, 0, c->stream>>>(s);
+    hipLaunchKernelGGL(k_screen, dim3(grid_for(n, 256, c->ncu * 16)), dim3(256), 0, c->stream, s);
     HIPOK(hipGetLastError());
   }
   for (auto& cp : c->classes) {
@@ -291,7 +291,7 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode) {
       if ((r = launch_class(c, cp.cls, prec, jonly, a, cp.cap))) return r;
     }
   }
-  k_add_sym<<<grid_for((long long)c->nb * c->nb, 256, 2048), 256, 0, c->stream>>>(c->d_X, dF, c->nb);
+  hipLaunchKernelGGL(k_add_sym, dim3(grid_for((long long)c->nb * c->nb, 256, 2048)), dim3(256), 0, c->stream, c->d_X, dF, c->nb);
   HIPOK(hipGetLastError());
   HIPOK(hipEventRecord(c->ev1, c->stream));
   c->last_kind = 1;
@@ -323,7 +323,7 @@ int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initia
     s.nsh = c->nsh;
     s.check = initial ? 0 : 1;
     s.all_prims = kClassKernels[cls].all_prims;
-    k_screen_diag<<<grid_for(n, 256, 1024), 256, 0, c->stream>>>(s);
+    hipLaunchKernelGGL(k_screen_diag, dim3(grid_for(n, 256, 1024)), dim3(256), 0, c->stream, s);
     HIPOK(hipGetLastError());
     ClassArgs a = base_args(c, dP, thr);
     a.list = c->d_diag_list[t];
@@ -341,7 +341,7 @@ int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initia
     }
     if ((r = launch_class(c, cls, 0, false, a, n))) return r;
   }
-  k_add_sym<<<grid_for((long long)c->nb * c->nb, 256, 2048), 256, 0, c->stream>>>(c->d_X, dF, c->nb);
+  hipLaunchKernelGGL(k_add_sym, dim3(grid_for((long long)c->nb * c->nb, 256, 2048)), dim3(256), 0, c->stream, c->d_X, dF, c->nb);
   HIPOK(hipGetLastError());
   HIPOK(hipEventRecord(c->ev1, c->stream));
   c->last_kind = 2;

@@ -68,7 +68,7 @@ __device__ inline void wave_append(bool pred, int2 v, int2* list, int* count, un
   }
   // statistics: wave sums of primitive quartets and bra primitive iterations
   unsigned long long a = pred ? primq : 0ull, b = pred ? brap : 0ull;
-  for (int o = 32; o > 0; o >>= 1) {
+  for (int o = HFX_WAVE / 2; o > 0; o >>= 1) {
     a += __shfl_xor(a, o);
     b += __shfl_xor(b, o);
   }
@@ -91,7 +91,7 @@ __global__ void k_screen(const ScreenArgs s) {
   // Every lane of a wave must reach wave_append, so the loop bound is
   // rounded up to whole waves.
   const long long stride = (long long)gridDim.x * blockDim.x;
-  const long long nround = ((n + 63) / 64) * 64;
+  const long long nround = ((n + HFX_WAVE - 1) / HFX_WAVE) * HFX_WAVE;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < nround; idx += stride) {
     bool ok = idx < n;
     int i = 0, j = 0;
@@ -135,7 +135,7 @@ __global__ void k_screen(const ScreenArgs s) {
 // (not truncated by `last`), bound = schw^2 * max(maxdm(ab), maxdm(bb)/4, maxdm(aa)/4).
 // With check == 0 every pair is taken (fock_build_initial_diag.f90:31-35).
 __global__ void k_screen_diag(const DiagScreenArgs s) {
-  const int nround = ((s.npair + 63) / 64) * 64;
+  const int nround = ((s.npair + HFX_WAVE - 1) / HFX_WAVE) * HFX_WAVE;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nround; i += gridDim.x * blockDim.x) {
     bool ok = i < s.npair;
     int g = 0;

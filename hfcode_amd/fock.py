@@ -73,26 +73,30 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise HfxError("libhfx.so not built: run `python -m hfcode_amd.build`")
-        L = ctypes.CDLL(LIB_PATH)
-        dp, ip, vp = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), ctypes.c_void_p
-        L.hfx_ctx_create.argtypes = [ctypes.POINTER(MolDesc), ctypes.POINTER(vp)]
-        L.hfx_ctx_destroy.argtypes = [vp]
-        L.hfx_ctx_destroy.restype = None
-        L.hfx_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_double]
-        for n in ("hfx_fock_nd",):
-            getattr(L, n).argtypes = [vp, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int]
-        for n in ("hfx_fock_diag", "hfx_fock_initial_diag"):
-            getattr(L, n).argtypes = [vp, dp, dp, ctypes.c_int, ctypes.c_double]
-        L.hfx_fock_nd_device.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, vp]
-        L.hfx_fock_diag_device.argtypes = [vp, vp, vp, ctypes.c_double, vp]
-        L.hfx_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
-        L.hfx_get_schwarz.argtypes = [vp, ctypes.c_int, dp, ip, ip]
-        L.hfx_debug_blocks.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip, dp]
-        L.hfx_debug_blocks.restype = ctypes.c_longlong
-        L.hfx_pair_info.argtypes = [vp, ctypes.c_int, ip, ip]
-        L.hfx_version.restype = ctypes.c_char_p
-        _lib = L
+        _lib = configure(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+def configure(L):
+    """Declare the C-ABI signatures of include/hfx.h on a loaded library."""
+    dp, ip, vp = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), ctypes.c_void_p
+    L.hfx_ctx_create.argtypes = [ctypes.POINTER(MolDesc), ctypes.POINTER(vp)]
+    L.hfx_ctx_destroy.argtypes = [vp]
+    L.hfx_ctx_destroy.restype = None
+    L.hfx_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+    for n in ("hfx_fock_nd",):
+        getattr(L, n).argtypes = [vp, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    for n in ("hfx_fock_diag", "hfx_fock_initial_diag"):
+        getattr(L, n).argtypes = [vp, dp, dp, ctypes.c_int, ctypes.c_double]
+    L.hfx_fock_nd_device.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, vp]
+    L.hfx_fock_diag_device.argtypes = [vp, vp, vp, ctypes.c_double, vp]
+    L.hfx_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.hfx_get_schwarz.argtypes = [vp, ctypes.c_int, dp, ip, ip]
+    L.hfx_debug_blocks.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip, dp]
+    L.hfx_debug_blocks.restype = ctypes.c_longlong
+    L.hfx_pair_info.argtypes = [vp, ctypes.c_int, ip, ip]
+    L.hfx_version.restype = ctypes.c_char_p
+    return L
 
 
 def _check(rc):
