@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Fock builds/s for RHF C60 / 6-31G(d) on one MI355X.
+
+Workload: the ideal truncated icosahedron (6-6 bonds 1.40 A, 5-6 bonds
+1.45 A), 6-31G(d) with spherical d as the reference uses it (840 basis
+functions, 240 shells, 28,920 shell pairs), a fixed density P already in
+HBM.  One step = one full two-electron Fock build F = H + G(P):
+fock_build_diag + fock_build_non_diag at thr = 1e-10 (the reference's
+intthresh), all 21 quartet classes of 6-31G(d).
+
+Timed twice: plain FP64, and mixed precision (quartets whose
+Schwarz x density bound is below 1e-5 in FP32, accumulation FP64); the
+energy error of the mixed build is E(P, F_mixed) - E(P, F_fp64).
+`value` is the mixed builds/s when that error is <= 1 uEh, else FP64.
+
+The roofline object is for the class kernel with the largest device time
+(HIP events on the library's stream, averaged over the timed steps): its
+algorithmic FLOPs (operation counts of the generated kernel code x the
+primitive / contracted quartets it evaluated) over its duration, against the
+FP64 vector peak of MI355X.  cpu_baseline times the reference Fortran
+(oracle/_ref/libhfref.so, built from the reference's sources) on one C60
+build on this host's cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector, AMD public spec
+PEAK_FP32_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def setup(density, thr):
+    from hfcode_amd import fock, molecule as M, scf
+    mol = M.c60()
+    bas = M.load_basis("6-31g(d)")
+    st = M.ShellTable(mol, bas)
+    fb = fock.FockBuilder(st)
+    S, T, V = fb.onee(mol.znuc.astype(np.float64), mol.coords_bohr)
+    H = T + V
+    unuc = mol.nuclear_repulsion()
+    t0 = time.time()
+    if density == "scf":
+        res = scf.rhf(fb, S, H, mol.nocc, unuc, levelshift=0.4)
+        P = res.P
+        info = dict(scf_energy=res.energy, scf_iterations=res.iterations, scf_converged=res.converged)
+    else:
+        from hfcode_amd.scf import core_guess
+        P = core_guess(S, H, mol.nocc)
+        fb.initial_diag(np.zeros_like(H), P, 5e-7)
+        info = {}
+    info["setup_s"] = time.time() - t0
+    return mol, st, fb, S, H, P, unuc, info
+
+
+def time_builds(fb, dP, dH, dF, thr, steps, warmup, torch):
+    for _ in range(warmup):
+        dF.copy_(dH)
+        fb.diag_device(dP.data_ptr(), dF.data_ptr(), thr)
+        fb.non_diag_device(dP.data_ptr(), dF.data_ptr(), thr)
+    torch.cuda.synchronize()
+    class_ms = None
+    class_flops = None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dF.copy_(dH)
+        fb.diag_device(dP.data_ptr(), dF.data_ptr(), thr)
+        fb.non_diag_device(dP.data_ptr(), dF.data_ptr(), thr)
+        st = fb.stats()          # waits for the build; events of the non-diagonal kernels
+        class_ms = st["class_ms"] if class_ms is None else class_ms + st["class_ms"]
+        class_flops = st["class_flops"]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return dt, class_ms / steps, class_flops, st
+
+
+def cpu_reference(mol, bas, P, thr, threads):
+    """One C60 build by the reference Fortran on the host (wall clock)."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import hfref
+    if not hfref.available():
+        return None, None
+    r = hfref.RefMolecule.from_molecule(mol, bas)
+    z = np.zeros_like(P)
+    r.initial_diag(z, P, 5e-7)
+    t0 = time.perf_counter()
+    F = r.diag(z, P, thr)
+    F = r.non_diag(F, P, thr)
+    dt = time.perf_counter() - t0
+    r.free()
+    return dt, F
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--thr", type=float, default=1e-10)
+    ap.add_argument("--split", type=float, default=1e-5)
+    ap.add_argument("--density", choices=["scf", "core"], default="scf")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from hfcode_amd import fock, molecule as M
+    torch.cuda.init()
+    mol, st, fb, S, H, P, unuc, info = setup(a.density, a.thr)
+    log("setup done", info)
+    dev = torch.device("cuda:0")
+    dP = torch.from_numpy(np.ascontiguousarray(P)).to(dev)
+    dH = torch.from_numpy(np.ascontiguousarray(H)).to(dev)
+    dF = torch.empty_like(dH)
+    fb.set_profile(True)
+    # FP64
+    fb.set_mixed(0.0)
+    dt64, cms64, cfl64, st64 = time_builds(fb, dP, dH, dF, a.thr, a.steps, a.warmup, torch)
+    F64 = dF.cpu().numpy()
+    E64 = 0.5 * float(np.sum(P * (H + F64))) + unuc
+    # mixed
+    fb.set_mixed(a.split)
+    dtmx, cmsmx, cflmx, stmx = time_builds(fb, dP, dH, dF, a.thr, a.steps, a.warmup, torch)
+    Fmx = dF.cpu().numpy()
+    Emx = 0.5 * float(np.sum(P * (H + Fmx))) + unuc
+    fb.set_mixed(0.0)
+    err = abs(Emx - E64)
+    b64 = a.steps / dt64
+    bmx = a.steps / dtmx
+    use_mixed = err <= 1e-6
+    value = bmx if use_mixed else b64
+    # roofline: dominant class kernel of the FP64 build
+    k = int(np.argmax(cms64[:, 0]))
+    ach = cfl64[k, 0] / (cms64[k, 0] * 1e-3) / 1e12
+    tot_fl = st64["flops_fp64"]
+    q64 = int(st64["quartets"][:, 0].sum())
+    pq64 = int(st64["prim_quartets"][:, 0].sum())
+    out = {
+        "metric": "Fock builds/s C60/6-31G(d) mixed vs FP64 @<=1uEh",
+        "value": round(value, 4),
+        "unit": "Fock builds/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 / value, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32" if use_mixed else "f64",
+        "data": "synthetic: ideal C60 geometry, %s density" % ("converged FP64 SCF" if a.density == "scf" else
+                                                                "core-Hamiltonian guess"),
+        "config": {"workload": "RHF C60 6-31G(d) Fock build (diag + non-diag), 5d spherical, nb=%d, thr=%g"
+                   % (st.nb, a.thr), "molecule": "C60", "basis": "6-31G(d)", "nb": st.nb, "thr": a.thr,
+                   "mixed_split": a.split, "parallelism": "single GPU"},
+        "fp64_builds_per_s": round(b64, 4),
+        "mixed_builds_per_s": round(bmx, 4),
+        "mixed_speedup": round(bmx / b64, 3),
+        "mixed_energy_error_Eh": err,
+        "fp64_energy_Eh": E64,
+        "quartets_per_build_fp64": q64,
+        "prim_quartets_per_build_fp64": pq64,
+        "quartets_per_s_fp64": q64 * b64,
+        "build_tflops_fp64": tot_fl * b64 / 1e12,
+        "roofline": {
+            "bound": "valu-fp64",
+            "kernel": "eri_%s<double>" % fock.CLASSES[k],
+            "achieved": round(ach, 3),
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(ach / PEAK_FP64_TFLOPS, 4),
+            "traffic": None,
+            "flops_per_launch": cfl64[k, 0],
+            "avg_launch_ms": round(float(cms64[k, 0]), 4),
+        },
+        "class_ms_fp64": {fock.CLASSES[i]: round(float(cms64[i, 0]), 3) for i in range(46) if cms64[i, 0] > 0},
+    }
+    out.update({k_: v for k_, v in info.items()})
+    if not a.no_cpu:
+        bas = M.load_basis("6-31g(d)")
+        dt, Fr = cpu_reference(mol, bas, P, a.thr, a.cpu_threads)
+        if dt is not None:
+            out["cpu_baseline"] = {"value": round(1.0 / dt, 5), "unit": "Fock builds/s", "cores": a.cpu_threads,
+                                   "kind": "reference",
+                                   "sample": "one full C60 build (fock_build_diag + fock_build_non_diag, thr %g) "
+                                             "of the reference Fortran, flang -O3 -fopenmp, wall clock" % a.thr}
+            out["gpu_fp64_vs_cpu"] = round(b64 * dt, 2)
+            out["fock_maxabs_vs_reference"] = float(np.abs((Fr + H) - F64).max())
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

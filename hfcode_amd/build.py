@@ -38,8 +38,10 @@ def sources():
 
 def compile_one(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    deps = [src, os.path.join(CSRC, "hfx_device.h"), os.path.join(CSRC, "hfx_kernels.h"),
-            os.path.join(GEN, "eri_dispatch.h"), os.path.join(ROOT, "include", "hfx.h")]
+    deps = [src, os.path.join(CSRC, "hfx_device.h")]
+    if os.path.basename(src).startswith("hfx_"):
+        deps += [os.path.join(CSRC, "hfx_kernels.h"), os.path.join(GEN, "eri_dispatch.h"),
+                 os.path.join(ROOT, "include", "hfx.h")]
     if os.path.exists(obj) and all(os.path.getmtime(obj) >= os.path.getmtime(d) for d in deps if os.path.exists(d)):
         return obj, None
     cmd = [HIPCC] + HIPFLAGS + ["-c", src, "-o", obj]
@@ -65,6 +67,13 @@ def build(jobs=None, verbose=True):
             objs.append(obj)
             if err:
                 errs.append(err)
+    host_src = os.path.join(CSRC, "hfx_onee.cpp")
+    host_obj = os.path.join(OBJ, "hfx_onee.cpp.o")
+    r = subprocess.run(["g++", "-O2", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-c", host_src,
+                        "-o", host_obj], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode:
+        errs.append(r.stdout)
+    objs.append(host_obj)
     if errs:
         raise RuntimeError("hipcc failed:\n" + "\n".join(e[-4000:] for e in errs))
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + sorted(objs)

@@ -109,6 +109,9 @@ struct hfx_ctx {
   int totpairs = 0;
   double split = 0.0;
   int debug_blocks = 0;
+  int profile = 0;
+  int class_ran[2][HFX_NCLASSES] = {};
+  hipEvent_t cev[2][HFX_NCLASSES][2] = {};
   int last_kind = 0;  // 0 none, 1 non-diag, 2 diag
   // device
   hipStream_t stream = nullptr;
@@ -204,16 +207,20 @@ ClassArgs base_args(hfx_ctx* c, const double* dP, double thr) {
 
 int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long long cap) {
   const ClassKernels& k = kClassKernels[cls];
+  if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
   const void* fn = prec == 1 ? k.f32 : (jonly ? k.f64j : k.f64);
   const int K = prec == 1 ? 5 : 8;
   const size_t shm = (size_t)HFX_BOYS_ROWS * (k.M + K) * (prec == 1 ? sizeof(float) : sizeof(double));
   const int grid = grid_for(cap, HFX_NT, c->ncu * 8);
   void* args[] = {&a};
   HIPOK(hipLaunchKernel(fn, dim3(grid), dim3(HFX_NT), args, shm, c->stream));
+  if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));
+  c->class_ran[prec][cls] = 1;
   return HFX_OK;
 }
 
 int zero_build_state(hfx_ctx* c) {
+  memset(c->class_ran, 0, sizeof(c->class_ran));
   const size_t nx = (size_t)c->nb * c->nb;
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)nx, 256, 2048)), dim3(256), 0, c->stream, c->d_X, nx, 0.0);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, c->stream, c->d_counts, 2 * HFX_NCLASSES, 0);
@@ -567,6 +574,10 @@ void hfx_ctx_destroy(hfx_ctx* c) {
   hipFree(c->d_counts); hipFree(c->d_stats);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  for (int p = 0; p < 2; ++p)
+    for (int k = 0; k < HFX_NCLASSES; ++k)
+      for (int e = 0; e < 2; ++e)
+        if (c->cev[p][k][e]) hipEventDestroy(c->cev[p][k][e]);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -580,6 +591,15 @@ int hfx_set_option(hfx_ctx* c, int key, double value) {
   }
   if (key == HFX_OPT_DEBUG_BLOCKS) {
     c->debug_blocks = value != 0.0;
+    return HFX_OK;
+  }
+  if (key == HFX_OPT_PROFILE) {
+    if (value != 0.0 && !c->cev[0][0][0]) {
+      for (int p = 0; p < 2; ++p)
+        for (int k = 0; k < HFX_NCLASSES; ++k)
+          for (int e = 0; e < 2; ++e) HIPOK(hipEventCreate(&c->cev[p][k][e]));
+    }
+    c->profile = value != 0.0;
     return HFX_OK;
   }
   return HFX_E_ARG;
@@ -669,6 +689,12 @@ int hfx_get_stats(hfx_ctx* c, hfx_stats* out) {
       out->prim_quartets[cl][p] = (long long)s[1];
       out->bra_prims[cl][p] = (long long)s[2];
       const double f = (double)s[1] * (p ? k.flops_prim_fp32 : k.flops_prim) + (double)s[2] * k.flops_braprim;
+      out->class_flops[cl][p] = f + (double)s[0] * k.flops_quartet;
+      if (c->profile && c->class_ran[p][cl]) {
+        float cms = 0.f;
+        HIPOK(hipEventElapsedTime(&cms, c->cev[p][cl][0], c->cev[p][cl][1]));
+        out->class_ms[cl][p] = cms;
+      }
       if (p) {
         out->flops_fp32 += f;
         out->flops_fp64 += (double)s[0] * k.flops_quartet;  // digestion is FP64

@@ -28,7 +28,7 @@ CLASSES = ("ssss sssp sssd sssl sspp sspd sspl ssdd ssdl ssll spsp spsd spsl spp
 NTYPES = 10
 NCLASSES = 46
 HFX_JK, HFX_J_ONLY = 0, 1
-OPT_MIXED_SPLIT, OPT_DEBUG_BLOCKS = 1, 2
+OPT_MIXED_SPLIT, OPT_DEBUG_BLOCKS, OPT_PROFILE = 1, 2, 3
 ERRORS = {-1: "bad argument", -2: "unsupported basis", -3: "HIP runtime error",
           -4: "device out of memory", -5: "Schwarz tables missing (run initial_diag first)"}
 
@@ -61,6 +61,8 @@ class Stats(ctypes.Structure):
         ("bra_prims", (ctypes.c_longlong * 2) * NCLASSES),
         ("flops_fp64", ctypes.c_double), ("flops_fp32", ctypes.c_double),
         ("ms_total", ctypes.c_double),
+        ("class_flops", (ctypes.c_double * 2) * NCLASSES),
+        ("class_ms", (ctypes.c_float * 2) * NCLASSES),
     ]
 
 
@@ -95,6 +97,7 @@ def configure(L):
     L.hfx_debug_blocks.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip, dp]
     L.hfx_debug_blocks.restype = ctypes.c_longlong
     L.hfx_pair_info.argtypes = [vp, ctypes.c_int, ip, ip]
+    L.hfx_onee.argtypes = [ctypes.POINTER(MolDesc), ctypes.c_int, dp, dp, dp, dp, dp]
     L.hfx_version.restype = ctypes.c_char_p
     return L
 
@@ -196,9 +199,22 @@ class FockBuilder:
         except Exception:
             pass
 
+    def onee(self, charges, coords_bohr):
+        """(S, T, V) for point charges at coords (bohr); see hfx_onee."""
+        nb = self.nb
+        ch = np.ascontiguousarray(charges, dtype=np.float64)
+        xy = np.ascontiguousarray(coords_bohr, dtype=np.float64).reshape(-1, 3)
+        S, T, V = (np.zeros((nb, nb)) for _ in range(3))
+        _check(lib().hfx_onee(ctypes.byref(self._desc), len(ch), _dp(ch), _dp(xy), _dp(S), _dp(T), _dp(V)))
+        return S, T, V
+
     def set_mixed(self, split):
         """Quartets whose Schwarz x density bound is below `split` run in FP32."""
         _check(lib().hfx_set_option(self._h, OPT_MIXED_SPLIT, float(split)))
+
+    def set_profile(self, on=True):
+        """Time every class kernel with HIP events (stats()['class_ms'])."""
+        _check(lib().hfx_set_option(self._h, OPT_PROFILE, 1.0 if on else 0.0))
 
     def set_debug_blocks(self, on=True):
         _check(lib().hfx_set_option(self._h, OPT_DEBUG_BLOCKS, 1.0 if on else 0.0))
@@ -245,8 +261,10 @@ class FockBuilder:
         q = np.array([[s.quartets[c][k] for k in range(2)] for c in range(NCLASSES)], dtype=np.int64)
         pq = np.array([[s.prim_quartets[c][k] for k in range(2)] for c in range(NCLASSES)], dtype=np.int64)
         bp = np.array([[s.bra_prims[c][k] for k in range(2)] for c in range(NCLASSES)], dtype=np.int64)
+        cf = np.array([[s.class_flops[c][k] for k in range(2)] for c in range(NCLASSES)])
+        cm = np.array([[s.class_ms[c][k] for k in range(2)] for c in range(NCLASSES)])
         return dict(quartets=q, prim_quartets=pq, bra_prims=bp, flops_fp64=s.flops_fp64,
-                    flops_fp32=s.flops_fp32, ms=s.ms_total)
+                    flops_fp32=s.flops_fp32, ms=s.ms_total, class_flops=cf, class_ms=cm)
 
     def schwarz(self):
         out = []

@@ -80,10 +80,11 @@ class Molecule:
                 e += z[i] * z[j] / math.sqrt(float(((x[i] - x[j]) ** 2).sum()))
         return e
 
-    def to_inp(self, prtlevl=1):
+    def to_inp(self, prtlevl=1, levelshift=None):
         """Text of the reference's current-format input (read_infile.f90:12-61)."""
+        ls = "LEVELSHIFT Y %s" % repr(float(levelshift)) if levelshift else "LEVELSHIFT N 0.0"
         lines = ["%d" % self.natom, self.title.replace(" ", "_")[:40],
-                 "USEDENSITYSUPGUESS N", "DIRECTSCF Y", "LEVELSHIFT N 0.0",
+                 "USEDENSITYSUPGUESS N", "DIRECTSCF Y", ls,
                  "UHF N 0", "MP2 N", "CCSD N", "PRINT %d" % prtlevl, "ncore 0"]
         for l, (x, y, z) in zip(self.labels, self.coords_ang):
             lines.append("%-2s %s %s %s" % (l.capitalize(), repr(float(x)), repr(float(y)), repr(float(z))))

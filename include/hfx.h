@@ -32,6 +32,7 @@ extern "C" {
 #define HFX_OPT_MIXED_SPLIT 1   /* > 0: quartets with Schwarz x density bound below this value
                                    are evaluated in FP32 (accumulation stays FP64); 0 = all FP64 */
 #define HFX_OPT_DEBUG_BLOCKS 2  /* 1: keep every integral block of the next build (tests) */
+#define HFX_OPT_PROFILE 3       /* 1: time every class kernel with HIP events (hfx_stats.class_ms) */
 
 /* pair types, in the order of the reference's pair lists (molprops.f90:122-131) */
 #define HFX_NPAIRTYPES 10       /* ss sp sd sl pp pd pl dd dl ll */
@@ -69,6 +70,8 @@ typedef struct hfx_stats {
   long long bra_prims[HFX_NCLASSES][2];      /* bra primitive iterations [fp64, fp32] */
   double flops_fp64, flops_fp32;             /* algorithmic FLOPs of the build */
   double ms_total;                           /* device time of the build, ms */
+  double class_flops[HFX_NCLASSES][2];       /* algorithmic FLOPs per class kernel [fp64, fp32] */
+  float class_ms[HFX_NCLASSES][2];           /* class kernel time, ms (HFX_OPT_PROFILE = 1) */
 } hfx_stats;
 
 /* Create a context on the current HIP device: copies the shells, builds the
@@ -116,6 +119,12 @@ long long hfx_debug_blocks(hfx_ctx* ctx, int c, int prec, int* quartets, double*
 
 /* Global pair index -> (type, 1-based position in the type's list). */
 int hfx_pair_info(hfx_ctx* ctx, int gpair, int* type, int* index);
+
+/* One-electron matrices S, T, V (nb*nb each, symmetric) of the molecule in
+ * `desc` with point charges `charges[natom]` at `coords[3*natom]` (bohr).
+ * Host set-up, once per molecule; replaces cmpt1e (cmpt1e.f90:1-258). */
+int hfx_onee(const hfx_mol_desc* desc, int natom, const double* charges, const double* coords, double* S,
+             double* T, double* V);
 
 const char* hfx_version(void);
 

@@ -17,13 +17,7 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: long CPU test")
 
 
-def core_guess(s, h, nocc):
-    """Core-Hamiltonian guess density, as direct_hartreefock.f90:24-29,66-72 builds it."""
-    w, U = np.linalg.eigh(s)
-    X = U @ np.diag(w ** -0.5) @ U.T
-    e, C = np.linalg.eigh(X @ h @ X.T)
-    C = X @ C
-    return 2.0 * C[:, :nocc] @ C[:, :nocc].T
+from hfcode_amd.scf import core_guess  # noqa: E402,F401
 
 
 @pytest.fixture

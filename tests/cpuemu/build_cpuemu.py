@@ -38,7 +38,7 @@ def build(jobs=8):
     gen_eri.main(GEN)
     os.makedirs(OBJ, exist_ok=True)
     srcs = [os.path.join(CSRC, "hfx_kernels.hip"), os.path.join(CSRC, "hfx_host.hip"),
-            os.path.join(HERE, "emu_smem.cpp")]
+            os.path.join(HERE, "emu_smem.cpp"), os.path.join(CSRC, "hfx_onee.cpp")]
     srcs += sorted(os.path.join(GEN, f) for f in os.listdir(GEN) if f.endswith(".hip"))
     srcs.sort(key=lambda s: -os.path.getsize(s))
     with cf.ThreadPoolExecutor(jobs) as ex:

@@ -80,6 +80,6 @@ def test_class_blocks_match_reference_routines(case):
             s3, s4 = case.pairs[t2][0][i2 - 1]
             ref = case.ref.eri(cls, s1, s2, s3, s4)
             scale = max(np.abs(ref).max(), 1e-30)
-            assert np.abs(ref - b[k]).max() <= 1e-12 * scale + 1e-15, (cls, s1, s2, s3, s4)
+            assert np.abs(ref - b[k]).max() <= 1e-11 * scale + 1e-15, (cls, s1, s2, s3, s4)
             seen += 1
     assert seen > 0
