@@ -53,10 +53,18 @@ struct ClassArgs {
   const float* boys32;
   double* blocks;          // optional: dump of every block (625 per quartet)
   double* schw_out;        // optional: sqrt(max|(ab|ab)|) per bra pair
+  double* eblk;            // block kernels: integrals of quartets [q0, q1), quartet-major
   double thr;
   int nb;
   int diag;                // 0 off-diagonal build, 1 bra==ket build
+  int q0, q1;              // block kernels: chunk of the quartet list
 };
+
+// Wave-level barrier: LDS written by one lane is visible to the others.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 
 template <typename R> struct BoysTerms;
 template <> struct BoysTerms<double> { static constexpr int K = 8; };

@@ -110,6 +110,8 @@ struct hfx_ctx {
   double split = 0.0;
   int debug_blocks = 0;
   int profile = 0;
+  double* d_eblk = nullptr;     // integral blocks of the block class kernels
+  long long eblk_quartets = 0;  // chunk size (quartets) the buffer holds
   int class_ran[2][HFX_NCLASSES] = {};
   hipEvent_t cev[2][HFX_NCLASSES][2] = {};
   int last_kind = 0;  // 0 none, 1 non-diag, 2 diag
@@ -219,6 +221,56 @@ int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   return HFX_OK;
 }
 
+const long long kEblkChunk = 1 << 18;  // quartets per block-kernel chunk (1.3 GB at 625 doubles)
+
+// Heavy classes: block kernel (integrals to HBM) + wave-per-quartet digestion,
+// chunk by chunk over quartets [0, n).
+int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long long n, double* dump) {
+  const ClassKernels& k = kClassKernels[cls];
+  if (n <= 0) return HFX_OK;
+  if (!c->d_eblk) {
+    if (dalloc(&c->d_eblk, (size_t)kEblkChunk * 625)) return HFX_E_NOMEM;
+    c->eblk_quartets = kEblkChunk;
+  }
+  const void* fn = prec == 1 ? k.f32 : k.f64;
+  const int K = prec == 1 ? 5 : 8;
+  const size_t shm = (size_t)HFX_BOYS_ROWS * (k.M + K) * (prec == 1 ? sizeof(float) : sizeof(double));
+  if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
+  for (long long q0 = 0; q0 < n; q0 += c->eblk_quartets) {
+    const long long q1 = std::min(n, q0 + c->eblk_quartets);
+    ClassArgs b = a;
+    b.eblk = c->d_eblk;
+    b.q0 = (int)q0;
+    b.q1 = (int)q1;
+    void* args[] = {&b};
+    HIPOK(hipLaunchKernel(fn, dim3(grid_for(q1 - q0, HFX_NT, c->ncu * 8)), dim3(HFX_NT), args, shm, c->stream));
+    DigestArgs g;
+    memset(&g, 0, sizeof(g));
+    g.list = a.list;
+    g.count = a.count;
+    g.eblk = c->d_eblk;
+    g.pairs = c->d_pairs;
+    g.P = a.P;
+    g.X = a.X;
+    g.dump = dump;
+    g.schw_out = a.schw_out;
+    g.thr = a.thr;
+    g.nb = c->nb;
+    g.diag = a.diag;
+    g.jonly = jonly;
+    g.q0 = (int)q0;
+    g.q1 = (int)q1;
+    for (int i = 0; i < 4; ++i) g.dims[i] = k.dims[i];
+    g.swap_bra = k.swap_bra;
+    g.swap_ket = k.swap_ket;
+    hipLaunchKernelGGL(k_digest, dim3(grid_for(q1 - q0, 4, c->ncu * 16)), dim3(256), 0, c->stream, g);
+    HIPOK(hipGetLastError());
+  }
+  if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));
+  c->class_ran[prec][cls] = 1;
+  return HFX_OK;
+}
+
 int zero_build_state(hfx_ctx* c) {
   memset(c->class_ran, 0, sizeof(c->class_ran));
   const size_t nx = (size_t)c->nb * c->nb;
@@ -284,6 +336,15 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode) {
     hipLaunchKernelGGL(k_screen, dim3(grid_for(n, 256, c->ncu * 16)), dim3(256), 0, c->stream, s);
     HIPOK(hipGetLastError());
   }
+  // surviving-quartet counts of the block classes decide their chunking
+  int hcounts[2 * HFX_NCLASSES] = {0};
+  bool any_split = false;
+  for (auto& cp : c->classes)
+    if (cp.cap > 0 && kClassKernels[cp.cls].split) any_split = true;
+  if (any_split) {
+    HIPOK(hipMemcpyAsync(hcounts, c->d_counts, sizeof(hcounts), hipMemcpyDeviceToHost, c->stream));
+    HIPOK(hipStreamSynchronize(c->stream));
+  }
   for (auto& cp : c->classes) {
     if (cp.cap <= 0) continue;
     for (int prec = 0; prec < (mixed ? 2 : 1); ++prec) {
@@ -295,7 +356,11 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode) {
         if ((r = alloc_debug_blocks(c, cp, cp.cap, prec))) return r;
         a.blocks = prec ? cp.blocks32 : cp.blocks64;
       }
-      if ((r = launch_class(c, cp.cls, prec, jonly, a, cp.cap))) return r;
+      if (kClassKernels[cp.cls].split)
+        r = launch_split(c, cp.cls, prec, jonly, a, hcounts[prec * HFX_NCLASSES + cp.cls], a.blocks);
+      else
+        r = launch_class(c, cp.cls, prec, jonly, a, cp.cap);
+      if (r) return r;
     }
   }
   hipLaunchKernelGGL(k_add_sym, dim3(grid_for((long long)c->nb * c->nb, 256, 2048)), dim3(256), 0, c->stream, c->d_X, dF, c->nb);
@@ -346,7 +411,11 @@ int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initia
         a.blocks = cp->blocks64;
       }
     }
-    if ((r = launch_class(c, cls, 0, false, a, n))) return r;
+    if (kClassKernels[cls].split)
+      r = launch_split(c, cls, 0, false, a, n, a.blocks);
+    else
+      r = launch_class(c, cls, 0, false, a, n);
+    if (r) return r;
   }
   hipLaunchKernelGGL(k_add_sym, dim3(grid_for((long long)c->nb * c->nb, 256, 2048)), dim3(256), 0, c->stream, c->d_X, dF, c->nb);
   HIPOK(hipGetLastError());
@@ -571,7 +640,7 @@ void hfx_ctx_destroy(hfx_ctx* c) {
   }
   hipFree(c->d_pairs); hipFree(c->d_prims); hipFree(c->d_shells); hipFree(c->d_boys64); hipFree(c->d_boys32);
   hipFree(c->d_P); hipFree(c->d_F); hipFree(c->d_X); hipFree(c->d_maxdm); hipFree(c->d_schw_out);
-  hipFree(c->d_counts); hipFree(c->d_stats);
+  hipFree(c->d_counts); hipFree(c->d_stats); hipFree(c->d_eblk);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   for (int p = 0; p < 2; ++p)

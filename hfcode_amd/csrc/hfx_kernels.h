@@ -33,6 +33,22 @@ struct DiagScreenArgs {
   int first, npair, nsh, check, all_prims;
 };
 
+struct DigestArgs {
+  const int2* list;            // quartet list of the class
+  const int* count;            // its length
+  const double* eblk;          // integrals of quartets [q0, q1), quartet-major, oriented layout
+  const PairRec* pairs;
+  const double* P;
+  double* X;
+  double* dump;                // optional: blocks in the reference routine's layout, 625 per quartet
+  double* schw_out;            // optional: sqrt(max|e|) per bra pair (initial diagonal build)
+  double thr;
+  int nb, diag, jonly, q0, q1;
+  int dims[4];                 // components of the oriented shells A, B, C, D
+  int swap_bra, swap_ket;      // oriented A is the pair's second shell (bra / ket)
+};
+
+__global__ void k_digest(const DigestArgs d);
 __global__ void k_fill_f64(double* x, size_t n, double v);
 __global__ void k_fill_i32(int* x, int n, int v);
 __global__ void k_fill_u64(unsigned long long* x, int n);

@@ -161,6 +161,112 @@ __global__ void k_screen_diag(const DiagScreenArgs s) {
   }
 }
 
+// Digestion of integral blocks produced by the block class kernels: one wave
+// per quartet.  The block is staged in LDS, its max |e| decides the block test
+// (fock_build_non_diag.f90:69: skip when max|e| < thr), and the six J/K
+// contractions of the symmetric-case digestion (fock_build_non_diag.f90:
+// 2871-2955 in closed form) are spread over the lanes:
+//   X_ab += 2w sum_cd e P_cd,  X_cd += 2w sum_ab e P_ab,
+//   X_ac -= w/2 sum_bd e P_bd, X_ad -= w/2 sum_bc e P_bc,
+//   X_bc -= w/2 sum_ad e P_ad, X_bd -= w/2 sum_ac e P_ac,
+// w = 1/2 per pair made of one shell twice (1/2 more for bra == ket).
+// In the diagonal builds elements with |e| <= 1e-10 are dropped first
+// (fock_build_diag.f90:22).
+#define HFX_DIG_WAVES 4
+__global__ void __launch_bounds__(64 * HFX_DIG_WAVES) k_digest(const DigestArgs d) {
+  __shared__ double s_e[HFX_DIG_WAVES][625];
+  __shared__ double s_p[HFX_DIG_WAVES][150];
+  const int lane = threadIdx.x % HFX_WAVE;
+  const int wv = threadIdx.x / HFX_WAVE;
+  const int nA = d.dims[0], nB = d.dims[1], nC = d.dims[2], nD = d.dims[3];
+  const int ne = nA * nB * nC * nD;
+  const int qend = min(d.q1, *d.count);
+  double* se = s_e[wv];
+  double* sp = s_p[wv];
+  const int nwaves = gridDim.x * (blockDim.x / HFX_WAVE);
+  for (int iq = d.q0 + blockIdx.x * (blockDim.x / HFX_WAVE) + wv; iq < qend; iq += nwaves) {
+    const int2 bk = d.list[iq];
+    const PairRec pb = d.pairs[bk.x];
+    const PairRec pk = d.pairs[bk.y];
+    const double* blk = d.eblk + (size_t)(iq - d.q0) * ne;
+    double vmax = 0.0;
+    for (int i = lane; i < ne; i += HFX_WAVE) {
+      double v = blk[i];
+      vmax = fmax(vmax, fabs(v));
+      if (d.diag && fabs(v) <= HFX_INTTHRESH) v = 0.0;
+      se[i] = v;
+      if (d.dump) {
+        const int id = i % nD, ic = (i / nD) % nC, ib = (i / (nD * nC)) % nB, ia = i / (nD * nC * nB);
+        const int i1 = d.swap_bra ? ib : ia, i2 = d.swap_bra ? ia : ib;
+        const int i3 = d.swap_ket ? id : ic, i4 = d.swap_ket ? ic : id;
+        const int n1 = d.swap_bra ? nB : nA, n2 = d.swap_bra ? nA : nB, n3 = d.swap_ket ? nD : nC;
+        d.dump[(size_t)iq * 625 + i1 + n1 * (i2 + n2 * (i3 + n3 * i4))] = v;
+      }
+    }
+    for (int o = HFX_WAVE / 2; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o));
+    if (d.schw_out && lane == 0) d.schw_out[bk.x] = sqrt(vmax);
+    if (!d.diag && vmax < d.thr) continue;
+    const int shA = d.swap_bra ? pb.shb : pb.sha, shB = d.swap_bra ? pb.sha : pb.shb;
+    const int shC = d.swap_ket ? pk.shb : pk.sha, shD = d.swap_ket ? pk.sha : pk.shb;
+    const int aoA = d.swap_bra ? pb.aob : pb.aoa, aoB = d.swap_bra ? pb.aoa : pb.aob;
+    const int aoC = d.swap_ket ? pk.aob : pk.aoa, aoD = d.swap_ket ? pk.aoa : pk.aob;
+    // density blocks: ab, cd, ac, ad, bc, bd
+    const int o_ab = 0, o_cd = o_ab + nA * nB, o_ac = o_cd + nC * nD, o_ad = o_ac + nA * nC;
+    const int o_bc = o_ad + nA * nD, o_bd = o_bc + nB * nC, np = o_bd + nB * nD;
+    for (int i = lane; i < np; i += HFX_WAVE) {
+      int r, c;
+      if (i < o_cd) { r = aoA + (i - o_ab) / nB; c = aoB + (i - o_ab) % nB; }
+      else if (i < o_ac) { r = aoC + (i - o_cd) / nD; c = aoD + (i - o_cd) % nD; }
+      else if (i < o_ad) { r = aoA + (i - o_ac) / nC; c = aoC + (i - o_ac) % nC; }
+      else if (i < o_bc) { r = aoA + (i - o_ad) / nD; c = aoD + (i - o_ad) % nD; }
+      else if (i < o_bd) { r = aoB + (i - o_bc) / nC; c = aoC + (i - o_bc) % nC; }
+      else { r = aoB + (i - o_bd) / nD; c = aoD + (i - o_bd) % nD; }
+      sp[i] = d.P[(size_t)r * d.nb + c];
+    }
+    wave_sync();
+    const double w = ((shA == shB) ? 0.5 : 1.0) * ((shC == shD) ? 0.5 : 1.0) * (d.diag ? 0.5 : 1.0);
+    const int nout = d.jonly ? o_ac : np;
+    for (int o = lane; o < nout; o += HFX_WAVE) {
+      double acc = 0.0;
+      int row, col;
+      double wt;
+      if (o < o_cd) {          // J_ab
+        const int a = (o - o_ab) / nB, b = (o - o_ab) % nB;
+        for (int c = 0; c < nC; ++c)
+          for (int e = 0; e < nD; ++e) acc += se[((a * nB + b) * nC + c) * nD + e] * sp[o_cd + c * nD + e];
+        row = aoA + a; col = aoB + b; wt = 2.0 * w;
+      } else if (o < o_ac) {   // J_cd
+        const int c = (o - o_cd) / nD, e = (o - o_cd) % nD;
+        for (int a = 0; a < nA; ++a)
+          for (int b = 0; b < nB; ++b) acc += se[((a * nB + b) * nC + c) * nD + e] * sp[o_ab + a * nB + b];
+        row = aoC + c; col = aoD + e; wt = 2.0 * w;
+      } else if (o < o_ad) {   // K_ac
+        const int a = (o - o_ac) / nC, c = (o - o_ac) % nC;
+        for (int b = 0; b < nB; ++b)
+          for (int e = 0; e < nD; ++e) acc += se[((a * nB + b) * nC + c) * nD + e] * sp[o_bd + b * nD + e];
+        row = aoA + a; col = aoC + c; wt = -0.5 * w;
+      } else if (o < o_bc) {   // K_ad
+        const int a = (o - o_ad) / nD, e = (o - o_ad) % nD;
+        for (int b = 0; b < nB; ++b)
+          for (int c = 0; c < nC; ++c) acc += se[((a * nB + b) * nC + c) * nD + e] * sp[o_bc + b * nC + c];
+        row = aoA + a; col = aoD + e; wt = -0.5 * w;
+      } else if (o < o_bd) {   // K_bc
+        const int b = (o - o_bc) / nC, c = (o - o_bc) % nC;
+        for (int a = 0; a < nA; ++a)
+          for (int e = 0; e < nD; ++e) acc += se[((a * nB + b) * nC + c) * nD + e] * sp[o_ad + a * nD + e];
+        row = aoB + b; col = aoC + c; wt = -0.5 * w;
+      } else {                 // K_bd
+        const int b = (o - o_bd) / nD, e = (o - o_bd) % nD;
+        for (int a = 0; a < nA; ++a)
+          for (int c = 0; c < nC; ++c) acc += se[((a * nB + b) * nC + c) * nD + e] * sp[o_ac + a * nC + c];
+        row = aoB + b; col = aoD + e; wt = -0.5 * w;
+      }
+      atomicAdd(&d.X[(size_t)row * d.nb + col], wt * acc);
+    }
+    wave_sync();
+  }
+}
+
 // F += X + X^T (the accumulator holds each symmetric contribution once).
 __global__ void k_add_sym(const double* __restrict__ X, double* __restrict__ F, int nb) {
   const long long n = (long long)nb * nb;

@@ -130,6 +130,50 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def build_desc(shells, pairs=None, nb=None):
+    """hfx_mol_desc for `shells` (molprops-style arrays) and optional pair
+    lists / Schwarz tables; returns (desc, keep-alive list, nb, pair lists)."""
+    g = (lambda k: shells[k]) if isinstance(shells, dict) else (lambda k: getattr(shells, k))
+    keep = []
+    shtyp = np.ascontiguousarray(g("shtyp"), dtype=np.int32)
+    nsh = len(shtyp)
+    zeta = np.ascontiguousarray(g("zeta"), dtype=np.float64)
+    maxprim = zeta.shape[1]
+    arrs = dict(
+        shtyp=shtyp,
+        nzet=np.ascontiguousarray(g("nzet"), dtype=np.int32),
+        aosh=np.ascontiguousarray(g("aosh"), dtype=np.int32),
+        xsh=np.ascontiguousarray(g("xsh"), dtype=np.float64).reshape(nsh, 3),
+        zeta=zeta,
+        djk=np.ascontiguousarray(g("djk"), dtype=np.float64),
+        djkl=np.ascontiguousarray(g("djkl"), dtype=np.float64),
+    )
+    keep.append(arrs)
+    ncomp = {1: 1, 2: 3, 3: 5, 12: 4}
+    nb = int(nb) if nb is not None else int(max(a - 1 + ncomp[int(t)] for a, t in zip(arrs["aosh"], shtyp)))
+    d = MolDesc()
+    d.nb, d.nsh, d.maxprim = nb, nsh, maxprim
+    d.shtyp, d.nzet, d.aosh = _ip(arrs["shtyp"]), _ip(arrs["nzet"]), _ip(arrs["aosh"])
+    d.xsh, d.zeta, d.djk, d.djkl = _dp(arrs["xsh"]), _dp(arrs["zeta"]), _dp(arrs["djk"]), _dp(arrs["djkl"])
+    if pairs is None:
+        pairs = [(p, None, None, 0) for p in pair_lists(shtyp)]
+    lists = []
+    for t, (prs, schw, order, last) in enumerate(pairs):
+        prs = np.ascontiguousarray(prs, dtype=np.int32).reshape(-1, 2)
+        lists.append(prs)
+        d.npair[t] = len(prs)
+        if len(prs) == 0:
+            continue
+        keep.append(prs)
+        d.prs[t] = _ip(prs)
+        if schw is not None:
+            s = np.ascontiguousarray(schw, dtype=np.float64)
+            o = np.ascontiguousarray(order, dtype=np.int32)
+            keep += [s, o]
+            d.schw[t], d.order[t], d.last[t] = _dp(s), _ip(o), int(last)
+    return d, keep, nb, lists
+
+
 class FockBuilder:
     """One molecule on the GPU.
 
@@ -144,48 +188,10 @@ class FockBuilder:
 
     def __init__(self, shells, pairs=None, nb=None):
         L = lib()
-        g = (lambda k: shells[k]) if isinstance(shells, dict) else (lambda k: getattr(shells, k))
-        self._keep = []
-        shtyp = np.ascontiguousarray(g("shtyp"), dtype=np.int32)
-        nsh = len(shtyp)
-        zeta = np.ascontiguousarray(g("zeta"), dtype=np.float64)
-        maxprim = zeta.shape[1]
-        arrs = dict(
-            shtyp=shtyp,
-            nzet=np.ascontiguousarray(g("nzet"), dtype=np.int32),
-            aosh=np.ascontiguousarray(g("aosh"), dtype=np.int32),
-            xsh=np.ascontiguousarray(g("xsh"), dtype=np.float64).reshape(nsh, 3),
-            zeta=zeta,
-            djk=np.ascontiguousarray(g("djk"), dtype=np.float64),
-            djkl=np.ascontiguousarray(g("djkl"), dtype=np.float64),
-        )
-        self._keep.append(arrs)
-        ncomp = {1: 1, 2: 3, 3: 5, 12: 4}
-        self.nb = int(nb) if nb is not None else int(max(a - 1 + ncomp[int(t)] for a, t in zip(arrs["aosh"], shtyp)))
-        self.nsh = nsh
-        d = MolDesc()
-        d.nb, d.nsh, d.maxprim = self.nb, nsh, maxprim
-        d.shtyp, d.nzet, d.aosh = _ip(arrs["shtyp"]), _ip(arrs["nzet"]), _ip(arrs["aosh"])
-        d.xsh, d.zeta, d.djk, d.djkl = _dp(arrs["xsh"]), _dp(arrs["zeta"]), _dp(arrs["djk"]), _dp(arrs["djkl"])
-        if pairs is None:
-            pairs = [(p, None, None, 0) for p in pair_lists(shtyp)]
-        self.pair_lists = []
-        for t, (prs, schw, order, last) in enumerate(pairs):
-            prs = np.ascontiguousarray(prs, dtype=np.int32).reshape(-1, 2)
-            self.pair_lists.append(prs)
-            d.npair[t] = len(prs)
-            if len(prs) == 0:
-                continue
-            self._keep.append(prs)
-            d.prs[t] = _ip(prs)
-            if schw is not None:
-                s = np.ascontiguousarray(schw, dtype=np.float64)
-                o = np.ascontiguousarray(order, dtype=np.int32)
-                self._keep += [s, o]
-                d.schw[t], d.order[t], d.last[t] = _dp(s), _ip(o), int(last)
-        self._desc = d
+        self._desc, self._keep, self.nb, self.pair_lists = build_desc(shells, pairs, nb)
+        self.nsh = self._desc.nsh
         h = ctypes.c_void_p()
-        _check(L.hfx_ctx_create(ctypes.byref(d), ctypes.byref(h)))
+        _check(L.hfx_ctx_create(ctypes.byref(self._desc), ctypes.byref(h)))
         self._h = h
 
     def close(self):

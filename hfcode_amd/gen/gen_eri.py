@@ -183,100 +183,82 @@ class Gen:
         return out
 
     # ---- code emission ----------------------------------------------------
-    def kernel(self):
-        L = []
-        W = L.append
-        cls = self.cls
-        ops_prim, ops_bra, ops_ctr = Ops(), Ops(), Ops()
-        M = self.M
-
-        # union of e (bra) and f (ket) Cartesian targets
-        e_all = sorted({l for bp in self.bra_parts for l in self.erange(bp)[2]})
-        f_by_kp = {kp: self.frange(kp)[2] for kp in self.ket_parts}
-        e_by_bp = {bp: self.erange(bp)[2] for bp in self.bra_parts}
-        targets = []
-        for kp in self.ket_parts:
-            for le in e_all:
-                for lf in f_by_kp[kp]:
-                    for e in cart(le):
-                        for f in cart(lf):
-                            if (e, f) not in targets:
-                                targets.append((e, f))
-        two_level = self.has_L_bra
-
+    def weights(self):
         def wket(kp):
-            terms = []
+            t = []
             if PARTS[self.C][kp[0]][1]:
-                terms.append("wC")
+                t.append("wC")
             if PARTS[self.D][kp[1]][1]:
-                terms.append("wD")
-            return "*".join(terms)
+                t.append("wD")
+            return "*".join(t)
 
         def wbra(bp):
-            terms = []
+            t = []
             if PARTS[self.A][bp[0]][1]:
-                terms.append("wA")
+                t.append("wA")
             if PARTS[self.B][bp[1]][1]:
-                terms.append("wB")
-            return "*".join(terms)
+                t.append("wB")
+            return "*".join(t)
+        return wbra, wket
 
-        def acc_name(bp, kp, e, f):
-            return "acc_%d%d_%d%d_%s_%s" % (bp[0], bp[1], kp[0], kp[1], tname(e), tname(f))
-
-        def tmp_name(kp, e, f):
-            return "tmp_%d%d_%s_%s" % (kp[0], kp[1], tname(e), tname(f))
-
-        # accumulators
-        acc_list = []
-        for bp in self.bra_parts:
-            for kp in self.ket_parts:
-                for le in e_by_bp[bp]:
-                    for lf in f_by_kp[kp]:
-                        for e in cart(le):
-                            for f in cart(lf):
-                                acc_list.append(acc_name(bp, kp, e, f))
-        tmp_list = []
-        if two_level:
-            for kp in self.ket_parts:
-                for le in e_all:
-                    for lf in f_by_kp[kp]:
-                        for e in cart(le):
-                            for f in cart(lf):
-                                tmp_list.append(tmp_name(kp, e, f))
-
-        oa = "b" if self.swap_bra else "a"    # PairRec field suffix for oriented A
+    def emit_quartet_setup(self, W):
+        oa = "b" if self.swap_bra else "a"
         ob = "a" if self.swap_bra else "b"
         oc = "b" if self.swap_ket else "a"
         od = "a" if self.swap_ket else "b"
-        wa_f = "wb" if self.swap_bra else "wa"
-        wb_f = "wa" if self.swap_bra else "wb"
-        wc_f = "wb" if self.swap_ket else "wa"
-        wd_f = "wa" if self.swap_ket else "wb"
-
-        W("// (%s|%s) generated by hfcode_amd/gen/gen_eri.py -- do not edit" % (cls[:2], cls[2:]))
-        W("// build centres: A=%s B=%s C=%s D=%s; Boys m <= %d" % (self.A, self.B, self.C, self.D, M))
-        W("#include \"hfx_device.h\"")
-        W("namespace hfx {")
-        W("template <typename R, bool JONLY>")
-        W("__global__ void __launch_bounds__(HFX_NT) eri_%s(const ClassArgs a) {" % cls)
-        W("  extern __shared__ char hfx_smem[];")
-        W("  R* btab = reinterpret_cast<R*>(hfx_smem);")
-        W("  constexpr int NB_COL = %d + BoysTerms<R>::K;" % M)
-        W("  stage_boys<R>(btab, a, NB_COL);")
-        W("  const int nq = *a.count;")
-        W("  const int nbf = a.nb;")
-        W("  for (int iq = blockIdx.x * blockDim.x + threadIdx.x; iq < nq; iq += gridDim.x * blockDim.x) {")
-        W("    const int2 bk = a.list[iq];")
         W("    const PairRec& pb = a.pairs[bk.x];")
         W("    const PairRec& pk = a.pairs[bk.y];")
         W("    const double Ax = pb.%sx, Ay = pb.%sy, Az = pb.%sz;" % (oa, oa, oa))
         W("    const double Cx = pk.%sx, Cy = pk.%sy, Cz = pk.%sz;" % (oc, oc, oc))
         W("    const R ABx = (R)(Ax - pb.%sx), ABy = (R)(Ay - pb.%sy), ABz = (R)(Az - pb.%sz);" % (ob, ob, ob))
         W("    const R CDx = (R)(Cx - pk.%sx), CDy = (R)(Cy - pk.%sy), CDz = (R)(Cz - pk.%sz);" % (od, od, od))
+        W("    (void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz;")
         W("    const int shA = pb.sh%s, shB = pb.sh%s, shC = pk.sh%s, shD = pk.sh%s;" % (oa, ob, oc, od))
         W("    const int aoA = pb.ao%s, aoB = pb.ao%s, aoC = pk.ao%s, aoD = pk.ao%s;" % (oa, ob, oc, od))
-        for n in acc_list:
-            W("    R %s = 0;" % n)
+        W("    (void)shA; (void)shB; (void)shC; (void)shD; (void)aoA; (void)aoB; (void)aoC; (void)aoD;")
+
+    def emit_group(self, W, combos, ops_prim, ops_bra, ops_ctr, gid):
+        """Primitive loops + contraction + HRR + spherical transform for the
+        part combinations `combos`; returns {(ia,ib,ic,id): element var}."""
+        wbra, wket = self.weights()
+        bps = sorted({c[0] for c in combos})
+        kps_by_bp = {bp: [c[1] for c in combos if c[0] == bp] for bp in bps}
+        kps = sorted({c[1] for c in combos})
+        e_by_bp = {bp: self.erange(bp)[2] for bp in bps}
+        f_by_kp = {kp: self.frange(kp)[2] for kp in kps}
+        e_all = sorted({l for bp in bps for l in e_by_bp[bp]})
+        two_level = len(bps) > 1 or (bps and wbra(bps[0]))
+        # a single bra part with a weight folds the weight into the ket sum
+        fold_bra = (not two_level) or False
+        M = max(max(e_by_bp[bp]) for bp in bps) + max(max(f_by_kp[kp]) for kp in kps)
+        targets = []
+        seen = set()
+        for kp in kps:
+            es = e_all if two_level else e_by_bp[bps[0]]
+            for le in es:
+                for lf in f_by_kp[kp]:
+                    for e in cart(le):
+                        for f in cart(lf):
+                            if (e, f) not in seen:
+                                seen.add((e, f))
+                                targets.append((e, f))
+
+        def acc_name(bp, kp, e, f):
+            return "acc%s_%d%d_%d%d_%s_%s" % (gid, bp[0], bp[1], kp[0], kp[1], tname(e), tname(f))
+
+        def tmp_name(kp, e, f):
+            return "tmp%s_%d%d_%s_%s" % (gid, kp[0], kp[1], tname(e), tname(f))
+
+        for (bp, kp) in combos:
+            for le in e_by_bp[bp]:
+                for lf in f_by_kp[kp]:
+                    for e in cart(le):
+                        for f in cart(lf):
+                            W("    R %s = 0;" % acc_name(bp, kp, e, f))
+        wa_f = "wb" if self.swap_bra else "wa"
+        wb_f = "wa" if self.swap_bra else "wb"
+        wc_f = "wb" if self.swap_ket else "wa"
+        wd_f = "wa" if self.swap_ket else "wb"
         npb = "pb.npall" if self.all_prims else "pb.np"
         npk = "pk.npall" if self.all_prims else "pk.np"
         W("    for (int ib = 0; ib < %s; ++ib) {" % npb)
@@ -288,8 +270,13 @@ class Gen:
             W("      const R wA = (R)rb.%s, wB = (R)rb.%s; (void)wA; (void)wB;" % (wa_f, wb_f))
         W("      const R oo2p = (R)0.5 * pinv; (void)oo2p;")
         ops_bra.n += 4
-        for n in tmp_list:
-            W("      R %s = 0;" % n)
+        if two_level:
+            for kp in kps:
+                for le in e_all:
+                    for lf in f_by_kp[kp]:
+                        for e in cart(le):
+                            for f in cart(lf):
+                                W("      R %s = 0;" % tmp_name(kp, e, f))
         W("      for (int ik = 0; ik < %s; ++ik) {" % npk)
         W("        const PrimRec& rk = a.prims[pk.p0 + ik];")
         W("        const R q = (R)rk.p, qinv = (R)rk.pinv, Kcd = (R)rk.K;")
@@ -305,7 +292,6 @@ class Gen:
         W("        const R WQx = rho_q * PQx, WQy = rho_q * PQy, WQz = rho_q * PQz;")
         W("        (void)WPx; (void)WPy; (void)WPz; (void)WQx; (void)WQy; (void)WQz;")
         W("        (void)QCx; (void)QCy; (void)QCz; (void)PAx; (void)PAy; (void)PAz;")
-        # ops: 1 add + div(1) + 2 mul + T(1 mul + 5 for dot) + 2 mul + 6 mul + 9 sub(PQ,QC)
         ops_prim.n += 1 + 1 + 2 + 6 + 2 + 6 + 6
         W("        R F[%d];" % (M + 1))
         W("        boys<R, %d>(btab, NB_COL, T, F);" % M)
@@ -320,8 +306,7 @@ class Gen:
         ops_prim.n += vops.n
         for ln in vlines:
             W("        " + ln)
-        # contraction
-        for kp in self.ket_parts:
+        for kp in kps:
             wk = wket(kp)
             if two_level:
                 for le in e_all:
@@ -334,7 +319,9 @@ class Gen:
                                 else:
                                     W("        %s += %s;" % (tmp_name(kp, e, f), v)); ops_prim.n += 1
             else:
-                bp = self.bra_parts[0]
+                bp = bps[0]
+                if kp not in kps_by_bp[bp]:
+                    continue
                 for le in e_by_bp[bp]:
                     for lf in f_by_kp[kp]:
                         for e in cart(le):
@@ -346,125 +333,144 @@ class Gen:
                                     W("        %s += %s;" % (acc_name(bp, kp, e, f), v)); ops_prim.n += 1
         W("      }")
         if two_level:
-            for bp in self.bra_parts:
+            for (bp, kp) in combos:
                 wb = wbra(bp)
-                for kp in self.ket_parts:
-                    for le in e_by_bp[bp]:
-                        for lf in f_by_kp[kp]:
-                            for e in cart(le):
-                                for f in cart(lf):
-                                    t = tmp_name(kp, e, f)
-                                    if wb:
-                                        W("      %s += %s * %s;" % (acc_name(bp, kp, e, f), wb, t)); ops_bra.n += 2
-                                    else:
-                                        W("      %s += %s;" % (acc_name(bp, kp, e, f), t)); ops_bra.n += 1
+                for le in e_by_bp[bp]:
+                    for lf in f_by_kp[kp]:
+                        for e in cart(le):
+                            for f in cart(lf):
+                                t = tmp_name(kp, e, f)
+                                if wb:
+                                    W("      %s += %s * %s;" % (acc_name(bp, kp, e, f), wb, t)); ops_bra.n += 2
+                                else:
+                                    W("      %s += %s;" % (acc_name(bp, kp, e, f), t)); ops_bra.n += 1
         W("    }")
-        # ---- HRR + spherical per part combination -> element expressions ----
-        elem = {}     # (ia, ib, ic, id) -> variable name
+        # ---- HRR + spherical per part combination ----
+        elem = {}
         cA, cB, cC, cD = comps(self.A), comps(self.B), comps(self.C), comps(self.D)
-        for bp in self.bra_parts:
+        for (bp, kp) in combos:
             la, lb, _ = self.erange(bp)
-            for kp in self.ket_parts:
-                lc, ld, fr = self.frange(kp)
-                tag = "%d%d%d%d" % (bp + kp)
-                fl = [f for lf in fr for f in cart(lf)]
-                # bra HRR
-                hm = {}
+            lc, ld, fr = self.frange(kp)
+            tag = "%s%d%d%d%d" % (gid, bp[0], bp[1], kp[0], kp[1])
+            fl = [f for lf in fr for f in cart(lf)]
+            hm = {}
 
-                def hb(a_, b_, f_):
-                    key = (a_, b_, f_)
-                    if key in hm:
-                        return hm[key]
-                    if sum(b_) == 0:
-                        hm[key] = acc_name(bp, kp, a_, f_)
-                        return hm[key]
-                    i = next(k for k in range(3) if b_[k] > 0)
-                    x1 = hb(add1(a_, i), sub1(b_, i), f_)
-                    x2 = hb(a_, sub1(b_, i), f_)
-                    nm = "hb%s_%s_%s_%s" % (tag, tname(a_), tname(b_), tname(f_))
-                    W("    const R %s = %s + AB%s * %s;" % (nm, x1, AX[i], x2))
-                    ops_ctr.n += 2
-                    hm[key] = nm
-                    return nm
+            def hb(a_, b_, f_, bp=bp, kp=kp, hm=hm, tag=tag):
+                key = (a_, b_, f_)
+                if key in hm:
+                    return hm[key]
+                if sum(b_) == 0:
+                    hm[key] = acc_name(bp, kp, a_, f_)
+                    return hm[key]
+                i = next(k for k in range(3) if b_[k] > 0)
+                x1 = hb(add1(a_, i), sub1(b_, i), f_)
+                x2 = hb(a_, sub1(b_, i), f_)
+                nm = "hb%s_%s_%s_%s" % (tag, tname(a_), tname(b_), tname(f_))
+                W("    const R %s = %s + AB%s * %s;" % (nm, x1, AX[i], x2))
+                ops_ctr.n += 2
+                hm[key] = nm
+                return nm
 
-                # bra components (after spherical) as functions of f
-                def bra_val(ca, cb, f_):
-                    # returns expression name for (ca cb| f_) with spherical handling
-                    def cart_terms(c, l):
-                        if c[1] == "c":
-                            return [(c[2], "1")]
-                        return SPH[c[2]]
-                    terms = []
-                    for (ta_, fa) in cart_terms(ca, la):
-                        for (tb_, fb) in cart_terms(cb, lb):
-                            terms.append((hb(ta_, tb_, f_), fa, fb))
-                    return terms
-
-                braname = {}
-                for ia, ca in enumerate(cA):
-                    if ca[0] != bp[0]:
+            braname = {}
+            for ia, ca in enumerate(cA):
+                if ca[0] != bp[0]:
+                    continue
+                for ib, cb in enumerate(cB):
+                    if cb[0] != bp[1]:
                         continue
-                    for ib, cb in enumerate(cB):
-                        if cb[0] != bp[1]:
+                    for f_ in fl:
+                        ta = [(ca[2], "1")] if ca[1] == "c" else SPH[ca[2]]
+                        tb = [(cb[2], "1")] if cb[1] == "c" else SPH[cb[2]]
+                        terms = [(hb(x, y, f_), fx, fy) for (x, fx) in ta for (y, fy) in tb]
+                        if len(terms) == 1 and terms[0][1] == "1" and terms[0][2] == "1":
+                            braname[(ia, ib, f_)] = terms[0][0]
+                        else:
+                            nm = "bs%s_%d_%d_%s" % (tag, ia, ib, tname(f_))
+                            ex = " + ".join("(R)(%s*%s) * %s" % (fx, fy, t) if (fx, fy) != ("1", "1") else t
+                                            for (t, fx, fy) in terms)
+                            W("    const R %s = %s;" % (nm, ex))
+                            ops_ctr.n += 2 * len(terms) - 1
+                            braname[(ia, ib, f_)] = nm
+            for ia, ca in enumerate(cA):
+                if ca[0] != bp[0]:
+                    continue
+                for ib, cb in enumerate(cB):
+                    if cb[0] != bp[1]:
+                        continue
+                    km = {}
+
+                    def hk(c_, d_, ia=ia, ib=ib, km=km, tag=tag, braname=braname):
+                        key = (c_, d_)
+                        if key in km:
+                            return km[key]
+                        if sum(d_) == 0:
+                            km[key] = braname[(ia, ib, c_)]
+                            return km[key]
+                        i = next(k for k in range(3) if d_[k] > 0)
+                        x1 = hk(add1(c_, i), sub1(d_, i))
+                        x2 = hk(c_, sub1(d_, i))
+                        nm = "hk%s_%d_%d_%s_%s" % (tag, ia, ib, tname(c_), tname(d_))
+                        W("    const R %s = %s + CD%s * %s;" % (nm, x1, AX[i], x2))
+                        ops_ctr.n += 2
+                        km[key] = nm
+                        return nm
+
+                    for ic, cc in enumerate(cC):
+                        if cc[0] != kp[0]:
                             continue
-                        for f_ in fl:
-                            terms = bra_val(ca, cb, f_)
+                        for id_, cd in enumerate(cD):
+                            if cd[0] != kp[1]:
+                                continue
+                            tc_ = [(cc[2], "1")] if cc[1] == "c" else SPH[cc[2]]
+                            td_ = [(cd[2], "1")] if cd[1] == "c" else SPH[cd[2]]
+                            terms = [(hk(x, y), fx, fy) for (x, fx) in tc_ for (y, fy) in td_]
+                            nm = "e_%d_%d_%d_%d" % (ia, ib, ic, id_)
                             if len(terms) == 1 and terms[0][1] == "1" and terms[0][2] == "1":
-                                braname[(ia, ib, f_)] = terms[0][0]
+                                W("    const R %s = %s;" % (nm, terms[0][0]))
                             else:
-                                nm = "bs%s_%d_%d_%s" % (tag, ia, ib, tname(f_))
-                                ex = " + ".join("(R)(%s*%s) * %s" % (fa, fb, t) if (fa, fb) != ("1", "1") else t
-                                                for (t, fa, fb) in terms)
+                                ex = " + ".join("(R)(%s*%s) * %s" % (fx, fy, t) if (fx, fy) != ("1", "1") else t
+                                                for (t, fx, fy) in terms)
                                 W("    const R %s = %s;" % (nm, ex))
                                 ops_ctr.n += 2 * len(terms) - 1
-                                braname[(ia, ib, f_)] = nm
-                # ket HRR for each bra component
-                for ia, ca in enumerate(cA):
-                    if ca[0] != bp[0]:
-                        continue
-                    for ib, cb in enumerate(cB):
-                        if cb[0] != bp[1]:
-                            continue
-                        km = {}
+                            elem[(ia, ib, ic, id_)] = nm
+        return elem
 
-                        def hk(c_, d_):
-                            key = (c_, d_)
-                            if key in km:
-                                return km[key]
-                            if sum(d_) == 0:
-                                km[key] = braname[(ia, ib, c_)]
-                                return km[key]
-                            i = next(k for k in range(3) if d_[k] > 0)
-                            x1 = hk(add1(c_, i), sub1(d_, i))
-                            x2 = hk(c_, sub1(d_, i))
-                            nm = "hk%s_%d_%d_%s_%s" % (tag, ia, ib, tname(c_), tname(d_))
-                            W("    const R %s = %s + CD%s * %s;" % (nm, x1, AX[i], x2))
-                            ops_ctr.n += 2
-                            km[key] = nm
-                            return nm
+    def ref_index(self, ia, ib, ic, id_):
+        n_ref = [NCOMP[x] for x in self.cls]
+        i1, i2 = (ib, ia) if self.swap_bra else (ia, ib)
+        i3, i4 = (id_, ic) if self.swap_ket else (ic, id_)
+        return i1 + n_ref[0] * (i2 + n_ref[1] * (i3 + n_ref[2] * i4))
 
-                        for ic, cc in enumerate(cC):
-                            if cc[0] != kp[0]:
-                                continue
-                            for id_, cd in enumerate(cD):
-                                if cd[0] != kp[1]:
-                                    continue
-                                tc_ = [(cc[2], "1")] if cc[1] == "c" else SPH[cc[2]]
-                                td_ = [(cd[2], "1")] if cd[1] == "c" else SPH[cd[2]]
-                                terms = [(hk(x, y), fx, fy) for (x, fx) in tc_ for (y, fy) in td_]
-                                nm = "e_%d_%d_%d_%d" % (ia, ib, ic, id_)
-                                if len(terms) == 1 and terms[0][1] == "1" and terms[0][2] == "1":
-                                    W("    const R %s = %s;" % (nm, terms[0][0]))
-                                else:
-                                    ex = " + ".join("(R)(%s*%s) * %s" % (fx, fy, t) if (fx, fy) != ("1", "1") else t
-                                                    for (t, fx, fy) in terms)
-                                    W("    const R %s = %s;" % (nm, ex))
-                                    ops_ctr.n += 2 * len(terms) - 1
-                                elem[(ia, ib, ic, id_)] = nm
-        nA, nB, nC, nD = len(cA), len(cB), len(cC), len(cD)
+    def dims(self):
+        return [NCOMP[x] for x in (self.A, self.B, self.C, self.D)]
+
+    def group_plan(self, split):
+        combos = [(bp, kp) for bp in self.bra_parts for kp in self.ket_parts]
+        if not split:
+            return [combos]
+        return [[c] for c in combos]
+
+    def kernel_fused(self, W, ops_prim, ops_bra, ops_ctr):
+        """One thread per quartet; integrals, block test and J/K digestion in
+        registers (light classes)."""
+        cls = self.cls
+        M = self.M
+        W("template <typename R, bool JONLY>")
+        W("__global__ void __launch_bounds__(HFX_NT) eri_%s(const ClassArgs a) {" % cls)
+        W("  extern __shared__ char hfx_smem[];")
+        W("  R* btab = reinterpret_cast<R*>(hfx_smem);")
+        W("  constexpr int NB_COL = %d + BoysTerms<R>::K;" % M)
+        W("  stage_boys<R>(btab, a, NB_COL);")
+        W("  const int nq = *a.count;")
+        W("  const int nbf = a.nb;")
+        W("  for (int iq = blockIdx.x * blockDim.x + threadIdx.x; iq < nq; iq += gridDim.x * blockDim.x) {")
+        W("    const int2 bk = a.list[iq];")
+        self.emit_quartet_setup(W)
+        elem = {}
+        for gi, combos in enumerate(self.group_plan(False)):
+            elem.update(self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%d" % gi))
+        nA, nB, nC, nD = self.dims()
         assert len(elem) == nA * nB * nC * nD
-        # ---- digestion -------------------------------------------------------
-        W("    // density blocks")
 
         def pblk(name, ao1, n1, ao2, n2, konly):
             for i in range(n1):
@@ -491,17 +497,12 @@ class Gen:
         W("    R vmax = 0;")
         W("    const R ecut = a.diag ? (R)HFX_INTTHRESH : (R)0;")
         W("    const bool dump = a.blocks != nullptr;")
-        # reference-orientation linear index for the debug block dump
-        n_ref = [NCOMP[x] for x in cls]
         for (ia, ib, ic, id_), nm in sorted(elem.items()):
             W("    {")
             W("      R v = %s;" % nm)
             W("      vmax = fmax(vmax, fabs(v));")
             W("      if (fabs(v) <= ecut) v = 0;")
-            i1, i2 = (ib, ia) if self.swap_bra else (ia, ib)
-            i3, i4 = (id_, ic) if self.swap_ket else (ic, id_)
-            lin = i1 + n_ref[0] * (i2 + n_ref[1] * (i3 + n_ref[2] * i4))
-            W("      if (dump) a.blocks[(size_t)iq * 625 + %d] = (double)v;" % lin)
+            W("      if (dump) a.blocks[(size_t)iq * 625 + %d] = (double)v;" % self.ref_index(ia, ib, ic, id_))
             W("      const double dv = (double)v;")
             W("      Jab_%d_%d = fma(dv, Pcd_%d_%d, Jab_%d_%d);" % (ia, ib, ic, id_, ia, ib))
             W("      Jcd_%d_%d = fma(dv, Pab_%d_%d, Jcd_%d_%d);" % (ic, id_, ia, ib, ic, id_))
@@ -534,18 +535,84 @@ class Gen:
         W("    }")
         W("  }")
         W("}")
-        W("template __global__ void eri_%s<double, false>(const ClassArgs);" % cls)
-        W("template __global__ void eri_%s<double, true>(const ClassArgs);" % cls)
-        W("template __global__ void eri_%s<float, false>(const ClassArgs);" % cls)
+
+    def kernel_block(self, W, ops_prim, ops_bra, ops_ctr):
+        """One thread per quartet of the chunk [a.q0, a.q1); the L-shell part
+        combinations are evaluated one after another and every integral is
+        stored to a.eblk (quartet-major, oriented layout); digestion follows
+        in k_digest (heavy classes)."""
+        cls = self.cls
+        M = self.M
+        nA, nB, nC, nD = self.dims()
+        W("template <typename R>")
+        W("__global__ void __launch_bounds__(HFX_NT) eri_%s_blk(const ClassArgs a) {" % cls)
+        W("  extern __shared__ char hfx_smem[];")
+        W("  R* btab = reinterpret_cast<R*>(hfx_smem);")
+        W("  constexpr int NB_COL = %d + BoysTerms<R>::K;" % M)
+        W("  stage_boys<R>(btab, a, NB_COL);")
+        W("  const int qend = min(a.q1, *a.count);")
+        W("  for (int iq = a.q0 + blockIdx.x * blockDim.x + threadIdx.x; iq < qend; iq += gridDim.x * blockDim.x) {")
+        W("    const int2 bk = a.list[iq];")
+        W("    double* out = a.eblk + (size_t)(iq - a.q0) * %d;" % (nA * nB * nC * nD))
+        self.emit_quartet_setup(W)
+        n = 0
+        for gi, combos in enumerate(self.group_plan(True)):
+            W("    {")
+            el = self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%d" % gi)
+            for (ia, ib, ic, id_), nm in sorted(el.items()):
+                W("    out[%d] = (double)%s;" % (((ia * nB + ib) * nC + ic) * nD + id_, nm))
+            n += len(el)
+            W("    }")
+        assert n == nA * nB * nC * nD
+        W("  }")
+        W("}")
+
+    def kernel(self, split):
+        L = []
+        W = L.append
+        ops_prim, ops_bra, ops_ctr = Ops(), Ops(), Ops()
+        W("// (%s|%s) generated by hfcode_amd/gen/gen_eri.py -- do not edit" % (self.cls[:2], self.cls[2:]))
+        W("// build centres: A=%s B=%s C=%s D=%s; Boys m <= %d; %s" % (
+            self.A, self.B, self.C, self.D, self.M, "block + k_digest" if split else "fused digestion"))
+        W("#include \"hfx_device.h\"")
+        W("namespace hfx {")
+        if split:
+            self.kernel_block(W, ops_prim, ops_bra, ops_ctr)
+            W("template __global__ void eri_%s_blk<double>(const ClassArgs);" % self.cls)
+            W("template __global__ void eri_%s_blk<float>(const ClassArgs);" % self.cls)
+            nA, nB, nC, nD = self.dims()
+            # digestion in k_digest: 6 contractions of every element + block max
+            ops_ctr.n += 13 * nA * nB * nC * nD
+        else:
+            self.kernel_fused(W, ops_prim, ops_bra, ops_ctr)
+            W("template __global__ void eri_%s<double, false>(const ClassArgs);" % self.cls)
+            W("template __global__ void eri_%s<double, true>(const ClassArgs);" % self.cls)
+            W("template __global__ void eri_%s<float, false>(const ClassArgs);" % self.cls)
         W("}  // namespace hfx")
+        ncombo = len(self.bra_parts) * len(self.ket_parts)
         info = {
-            "class": cls, "M": M, "bra": self.bra_type, "ket": self.ket_type,
+            "class": self.cls, "M": self.M, "bra": self.bra_type, "ket": self.ket_type,
+            "split": bool(split), "combos": ncombo,
             "flops_prim_vrr": ops_prim.n,
             "flops_braprim": ops_bra.n,
             "flops_quartet": ops_ctr.n,
             "all_prims": self.all_prims,
         }
         return "\n".join(L) + "\n", info
+
+    def fused_cost(self):
+        """Doubles a fused kernel keeps live: accumulators + J/K partial sums + density blocks."""
+        def nc(ls):
+            return sum((l + 1) * (l + 2) // 2 for l in ls)
+        e_all = sorted({l for bp in self.bra_parts for l in self.erange(bp)[2]})
+        acc = sum(nc(self.erange(bp)[2]) * nc(self.frange(kp)[2]) for bp in self.bra_parts for kp in self.ket_parts)
+        tmp = sum(nc(e_all) * nc(self.frange(kp)[2]) for kp in self.ket_parts) if self.has_L_bra else 0
+        nA, nB, nC, nD = self.dims()
+        dig = 2 * (nA * nB + nC * nD + nA * nC + nA * nD + nB * nC + nB * nD)
+        return acc + tmp + dig
+
+
+SPLIT_COST = 200    # classes whose fused kernel would keep more doubles live are split
 
 
 def boys_flops(M, K):
@@ -559,34 +626,45 @@ def main(outdir):
     table = {}
     names = []
     for cls in CLASSES:
-        src, info = Gen(cls).kernel()
+        g = Gen(cls)
+        split = g.fused_cost() > SPLIT_COST
+        src, info = g.kernel(split)
         path = os.path.join(outdir, "eri_%s.hip" % cls)
         old = open(path).read() if os.path.exists(path) else None
         if old != src:
             with open(path, "w") as fh:
                 fh.write(src)
-        info["flops_prim"] = info["flops_prim_vrr"] + boys_flops(info["M"], BOYS_TERMS["double"])
-        info["flops_prim_fp32"] = info["flops_prim_vrr"] + boys_flops(info["M"], BOYS_TERMS["float"])
+        info["flops_prim"] = info["flops_prim_vrr"] + (info["combos"] if split else 1) * boys_flops(
+            info["M"], BOYS_TERMS["double"])
+        info["flops_prim_fp32"] = info["flops_prim_vrr"] + (info["combos"] if split else 1) * boys_flops(
+            info["M"], BOYS_TERMS["float"])
+        info["dims"] = g.dims()
+        info["swap_bra"] = g.swap_bra
+        info["swap_ket"] = g.swap_ket
         table[cls] = info
         names.append(cls)
-    # dispatch table
     D = ["// generated by hfcode_amd/gen/gen_eri.py -- do not edit",
          "#pragma once", "#include \"hfx_device.h\"", "namespace hfx {"]
     for c in names:
-        D.append("template <typename R, bool JONLY> __global__ void eri_%s(const ClassArgs);" % c)
+        if table[c]["split"]:
+            D.append("template <typename R> __global__ void eri_%s_blk(const ClassArgs);" % c)
+        else:
+            D.append("template <typename R, bool JONLY> __global__ void eri_%s(const ClassArgs);" % c)
     D.append("struct ClassKernels {")
     D.append("  const void* f64; const void* f64j; const void* f32; int M; int all_prims;")
     D.append("  double flops_prim, flops_prim_fp32, flops_braprim, flops_quartet;")
+    D.append("  int split; int dims[4]; int swap_bra, swap_ket;")
     D.append("};")
     D.append("static const ClassKernels kClassKernels[%d] = {" % len(names))
     for c in names:
         t = table[c]
-        t["flops_prim"] = t["flops_prim_vrr"] + boys_flops(t["M"], BOYS_TERMS["double"])
-        t["flops_prim_fp32"] = t["flops_prim_vrr"] + boys_flops(t["M"], BOYS_TERMS["float"])
-        D.append("  {(const void*)&eri_%s<double, false>, (const void*)&eri_%s<double, true>, "
-                 "(const void*)&eri_%s<float, false>, %d, %d, %d, %d, %d, %d}," % (
-                     c, c, c, t["M"], int(t["all_prims"]), t["flops_prim"], t["flops_prim_fp32"],
-                     t["flops_braprim"], t["flops_quartet"]))
+        if t["split"]:
+            fns = "(const void*)&eri_%s_blk<double>, (const void*)&eri_%s_blk<double>, (const void*)&eri_%s_blk<float>" % (c, c, c)
+        else:
+            fns = "(const void*)&eri_%s<double, false>, (const void*)&eri_%s<double, true>, (const void*)&eri_%s<float, false>" % (c, c, c)
+        D.append("  {%s, %d, %d, %d, %d, %d, %d, %d, {%d, %d, %d, %d}, %d, %d}," % (
+            fns, t["M"], int(t["all_prims"]), t["flops_prim"], t["flops_prim_fp32"], t["flops_braprim"],
+            t["flops_quartet"], int(t["split"]), *t["dims"], int(t["swap_bra"]), int(t["swap_ket"])))
     D.append("};")
     D.append("}  // namespace hfx")
     path = os.path.join(outdir, "eri_dispatch.h")

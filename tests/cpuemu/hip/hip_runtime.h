@@ -45,6 +45,8 @@ static const hfx_emu_idx blockDim = {1, 1, 1};
 static const hfx_emu_idx gridDim = {1, 1, 1};
 
 inline void __syncthreads() {}
+#define __builtin_amdgcn_fence(order, scope) do { } while (0)
+#define __builtin_amdgcn_wave_barrier() do { } while (0)
 inline unsigned long long __ballot(bool p) { return p ? 1ull : 0ull; }
 inline int __lane_id() { return 0; }
 inline int __ffsll(long long m) { return m ? __builtin_ffsll(m) : 0; }
@@ -60,6 +62,8 @@ using std::fabs;
 using std::fma;
 using std::fmax;
 using std::sqrt;
+using std::min;
+using std::max;
 
 // dynamic LDS of the class kernels
 extern char hfx_smem[];
