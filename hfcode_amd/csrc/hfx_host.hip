@@ -292,6 +292,11 @@ int alloc_debug_blocks(hfx_ctx* c, ClassPlan& cp, long long cap, int prec) {
   double** b = prec ? &cp.blocks32 : &cp.blocks64;
   if (*b) { hipFree(*b); *b = nullptr; }
   if (cap > 0 && dalloc(b, (size_t)cap * 625)) return HFX_E_NOMEM;
+  if (cap > 0) {
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(cap * 625, 256, 4096)), dim3(256), 0, c->stream, *b,
+                       (size_t)cap * 625, 0.0);
+    HIPOK(hipGetLastError());
+  }
   return HFX_OK;
 }
 

@@ -75,8 +75,25 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise HfxError("libhfx.so not built: run `python -m hfcode_amd.build`")
+        _init_torch_first()
         _lib = configure(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+def _init_torch_first():
+    """PyTorch-ROCm carries its own copy of the HIP runtime, libhfx uses the
+    system one; both share the HSA runtime, and torch only finds the GPU when
+    its runtime initialises first.  Device tensors handed to the *_device
+    entry points come from torch, so torch is initialised before libhfx."""
+    try:
+        import torch
+    except ImportError:
+        return
+    try:
+        if torch.cuda.device_count() > 0:
+            torch.cuda.init()
+    except Exception:
+        pass
 
 
 def configure(L):
