@@ -22,19 +22,30 @@ CLASSES = ("ssss sssp sssd sssl sspp sspd sspl ssdd ssdl ssll spsp spsd spsl spp
            "sppl spdd spll sdsd sdsl sdpp sdpd sddd sddl sdll slsl slpp slpl sldd sldl "
            "slll pppp pppd pppl ppdd ppll pdpd pddd plpl plll dddd dddl ddll dldl dlll llll").split()
 
+DROPIN_PATH = os.path.join(_HERE, "_ref", "libhfref_hip.so")
 _lib = None
+_path = LIB_PATH
 
 
-def available():
-    return os.path.exists(LIB_PATH)
+def use_library(path):
+    """Select the library before the first call (e.g. DROPIN_PATH: the
+    reference with fock_build_non_diag replaced by the hfx Fortran shim)."""
+    global _path
+    if _lib is not None and path != _path:
+        raise RuntimeError("a reference library is already loaded in this process")
+    _path = path
+
+
+def available(path=None):
+    return os.path.exists(path or LIB_PATH)
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not available():
-            raise FileNotFoundError("%s not built (make -C oracle)" % LIB_PATH)
-        L = ctypes.CDLL(LIB_PATH)
+        if not os.path.exists(_path):
+            raise FileNotFoundError("%s not built (make -C oracle)" % _path)
+        L = ctypes.CDLL(_path)
         dp = ctypes.POINTER(ctypes.c_double)
         ip = ctypes.POINTER(ctypes.c_int)
         L.hfref_load.restype = ctypes.c_int
