@@ -155,4 +155,5 @@ def test_digestion_is_textbook_g_of_p(name):
                 for (u, du, au), (v, dv, av) in (((i, di, ai), (j, dj, aj)), ((j, dj, aj), (i, di, ai))):
                     blk[ax:ax + dx, ay:ay + dy, au:au + du, av:av + dv] = eri[ax:ax + dx, ay:ay + dy, au:au + du, av:av + dv]
             Jd += np.einsum("abcd,cd->ab", blk, P)
-    assert np.abs(F - (Jo - Jd)).max() < 1e-12
+    # the builds skip negligible primitive pairs (|K_ab|/sqrt(p) < 1e-12), the dense tensor does not
+    assert np.abs(F - (Jo - Jd)).max() < 1e-10
