@@ -53,7 +53,10 @@ def setup(density, thr):
     unuc = mol.nuclear_repulsion()
     t0 = time.time()
     if density == "scf":
-        res = scf.rhf(fb, S, H, mol.nocc, unuc, levelshift=0.4)
+        # superposition-of-atomic-densities guess, as the reference does with
+        # USEDENSITYSUPGUESS (direct_hartreefock.f90:50-64)
+        P0 = M.atomic_density_guess(mol, bas)
+        res = scf.rhf(fb, S, H, mol.nocc, unuc, P0=P0, maxiter=40)
         P = res.P
         info = dict(scf_energy=res.energy, scf_iterations=res.iterations, scf_converged=res.converged)
     else:

@@ -4,8 +4,9 @@ The `.dat` layout is the one `read_basfile` parses (read_basfile.f90:12-70):
 ``ELEMENTS n dmat`` then, per element, the shell count, the element label,
 and per shell ``T nprim scale`` followed by ``nprim`` lines of exponent and
 coefficient (plus a p coefficient for ``L`` shells), closed by ``****``.
-Optional atomic density blocks (``dmat=1``, read_basfile.f90:71-91) are not
-converted: the Fock build does not use them.
+With ``dmat=1`` the file ends with one atomic density block per element
+(read_basfile.f90:71-91, used by the superposition-of-atomic-densities
+guess, direct_hartreefock.f90:50-64); they are kept as ``density``.
 
 The JSON keeps the raw (unnormalised) published coefficients; normalisation is
 done by ``hfcode_amd.molecule.normalize`` exactly as read_basfile.f90:100-179
@@ -44,15 +45,34 @@ def parse_dat(text):
         assert toks[pos][0].startswith('****'), toks[pos]
         pos += 1
         elements[label] = shells
-    return elements
+    densities = {}
+    if int(head[2]) == 1 and pos < len(toks):
+        assert toks[pos][0].upper() == "DENSITY", toks[pos]
+        pos += 1
+        ncomp = {"s": 1, "p": 3, "d": 5, "l": 4}
+        for label in elements:
+            lab = toks[pos][0].lower(); pos += 1
+            assert lab == label, (lab, label)
+            n = sum(ncomp[sh["type"]] for sh in elements[label])
+            vals = []
+            while len(vals) < n * n:
+                vals += [float(v) for v in toks[pos]]
+                pos += 1
+            assert toks[pos][0].startswith('****'), toks[pos]
+            pos += 1
+            densities[label] = [vals[i * n:(i + 1) * n] for i in range(n)]
+    return elements, densities
 
 
 def main(argv):
     src, out, name = argv[1], argv[2], argv[3]
     with open(src) as fh:
-        elements = parse_dat(fh.read())
+        elements, densities = parse_dat(fh.read())
+    doc = {"name": name, "elements": elements}
+    if densities:
+        doc["density"] = densities
     with open(out, 'w') as fh:
-        json.dump({"name": name, "elements": elements}, fh, indent=1)
+        json.dump(doc, fh, indent=1)
 
 
 if __name__ == '__main__':

@@ -130,6 +130,7 @@ struct hfx_ctx {
   double* d_schw_out = nullptr;   // per global pair
   int* d_act[HFX_NPAIRTYPES] = {nullptr};        // active global pair indices (rank order)
   double* d_act_schw[HFX_NPAIRTYPES] = {nullptr};
+  int4* d_act_info[HFX_NPAIRTYPES] = {nullptr};   // shells and primitive counts per active position
   double* d_pair_schw[HFX_NPAIRTYPES] = {nullptr};  // Schwarz per pair (list order)
   int2* d_diag_list[HFX_NPAIRTYPES] = {nullptr};
   int* d_counts = nullptr;                     // [2][HFX_NCLASSES]
@@ -161,15 +162,22 @@ int upload_active(hfx_ctx* c) {
     if (n == 0) continue;
     std::vector<int> act(c->last[t]);
     std::vector<double> asc(c->last[t]);
+    std::vector<int4> info(c->last[t]);
     for (int k = 0; k < c->last[t]; ++k) {
       const int pi = c->order[t][k];
       act[k] = c->offset[t] + pi;
       asc[k] = c->schw[t][pi];
-      c->pairs[act[k]].rank = k;
+      PairRec& pr = c->pairs[act[k]];
+      pr.rank = k;
+      info[k] = make_int4(pr.sha, pr.shb, pr.np, pr.npall);
     }
     if (c->d_act[t]) { hipFree(c->d_act[t]); c->d_act[t] = nullptr; }
     if (c->d_act_schw[t]) { hipFree(c->d_act_schw[t]); c->d_act_schw[t] = nullptr; }
-    if (dalloc(&c->d_act[t], act.size()) || dalloc(&c->d_act_schw[t], asc.size())) return HFX_E_NOMEM;
+    if (c->d_act_info[t]) { hipFree(c->d_act_info[t]); c->d_act_info[t] = nullptr; }
+    if (dalloc(&c->d_act[t], act.size()) || dalloc(&c->d_act_schw[t], asc.size()) ||
+        dalloc(&c->d_act_info[t], info.size()))
+      return HFX_E_NOMEM;
+    HIPOK(hipMemcpy(c->d_act_info[t], info.data(), info.size() * sizeof(int4), hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(c->d_act[t], act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(c->d_act_schw[t], asc.data(), asc.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(c->d_pair_schw[t], c->schw[t].data(), n * sizeof(double), hipMemcpyHostToDevice));
@@ -232,9 +240,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     if (dalloc(&c->d_eblk, (size_t)kEblkChunk * 625)) return HFX_E_NOMEM;
     c->eblk_quartets = kEblkChunk;
   }
-  const void* fn = prec == 1 ? k.f32 : k.f64;
   const int K = prec == 1 ? 5 : 8;
-  const size_t shm = (size_t)HFX_BOYS_ROWS * (k.M + K) * (prec == 1 ? sizeof(float) : sizeof(double));
   if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
   for (long long q0 = 0; q0 < n; q0 += c->eblk_quartets) {
     const long long q1 = std::min(n, q0 + c->eblk_quartets);
@@ -243,7 +249,11 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     b.q0 = (int)q0;
     b.q1 = (int)q1;
     void* args[] = {&b};
-    HIPOK(hipLaunchKernel(fn, dim3(grid_for(q1 - q0, HFX_NT, c->ncu * 8)), dim3(HFX_NT), args, shm, c->stream));
+    for (int ic = 0; ic < k.ncombo; ++ic) {
+      const void* fn = prec == 1 ? k.c32[ic] : k.c64[ic];
+      const size_t shm = (size_t)HFX_BOYS_ROWS * (k.combo_M[ic] + K) * (prec == 1 ? sizeof(float) : sizeof(double));
+      HIPOK(hipLaunchKernel(fn, dim3(grid_for(q1 - q0, HFX_NT, c->ncu * 8)), dim3(HFX_NT), args, shm, c->stream));
+    }
     DigestArgs g;
     memset(&g, 0, sizeof(g));
     g.list = a.list;
@@ -319,6 +329,8 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode) {
     memset(&s, 0, sizeof(s));
     s.bra = c->d_act[cp.t1];
     s.ket = c->d_act[cp.t2];
+    s.binfo = c->d_act_info[cp.t1];
+    s.kinfo = c->d_act_info[cp.t2];
     s.schw_bra = c->d_act_schw[cp.t1];
     s.schw_ket = c->d_act_schw[cp.t2];
     s.pairs = c->d_pairs;
@@ -641,7 +653,7 @@ void hfx_ctx_destroy(hfx_ctx* c) {
     hipFree(cp.list64); hipFree(cp.list32); hipFree(cp.blocks64); hipFree(cp.blocks32);
   }
   for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
-    hipFree(c->d_act[t]); hipFree(c->d_act_schw[t]); hipFree(c->d_pair_schw[t]); hipFree(c->d_diag_list[t]);
+    hipFree(c->d_act[t]); hipFree(c->d_act_schw[t]); hipFree(c->d_act_info[t]); hipFree(c->d_pair_schw[t]); hipFree(c->d_diag_list[t]);
   }
   hipFree(c->d_pairs); hipFree(c->d_prims); hipFree(c->d_shells); hipFree(c->d_boys64); hipFree(c->d_boys32);
   hipFree(c->d_P); hipFree(c->d_F); hipFree(c->d_X); hipFree(c->d_maxdm); hipFree(c->d_schw_out);

@@ -7,6 +7,8 @@ namespace hfx {
 struct ScreenArgs {
   const int* bra;              // global pair index per bra list position
   const int* ket;
+  const int4* binfo;           // per list position: shell a, shell b, np, npall
+  const int4* kinfo;
   const double* schw_bra;      // Schwarz value per list position
   const double* schw_ket;
   const PairRec* pairs;

@@ -106,21 +106,21 @@ __global__ void k_screen(const ScreenArgs s) {
     if (ok) {
       gb = s.bra[i];
       gk = s.ket[j];
-      const PairRec b = s.pairs[gb];
-      const PairRec k = s.pairs[gk];
+      const int4 b = s.binfo[i];
+      const int4 k = s.kinfo[j];
       const int ns = s.nsh;
       const double* md = s.maxdm;
-      double mv = fmax(md[(size_t)b.sha * ns + b.shb], md[(size_t)k.sha * ns + k.shb]);
+      double mv = fmax(md[(size_t)b.x * ns + b.y], md[(size_t)k.x * ns + k.y]);
       if (!s.jonly) {
-        const double ex = fmax(fmax(md[(size_t)b.sha * ns + k.sha], md[(size_t)b.sha * ns + k.shb]),
-                               fmax(md[(size_t)b.shb * ns + k.sha], md[(size_t)b.shb * ns + k.shb]));
+        const double ex = fmax(fmax(md[(size_t)b.x * ns + k.x], md[(size_t)b.x * ns + k.y]),
+                               fmax(md[(size_t)b.y * ns + k.x], md[(size_t)b.y * ns + k.y]));
         mv = fmax(mv, 0.25 * ex);
       }
       const double bound = s.schw_bra[i] * s.schw_ket[j] * mv;
       if (!(bound < s.thr)) {
         if (s.split > 0.0 && bound < s.split) keep32 = true; else keep64 = true;
-        const int nb_ = s.all_prims ? b.npall : b.np;
-        const int nk_ = s.all_prims ? k.npall : k.np;
+        const int nb_ = s.all_prims ? b.w : b.z;
+        const int nk_ = s.all_prims ? k.w : k.z;
         primq = (unsigned long long)nb_ * (unsigned long long)nk_;
         brap = (unsigned long long)nb_;
       }

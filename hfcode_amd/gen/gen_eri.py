@@ -227,9 +227,9 @@ class Gen:
         e_by_bp = {bp: self.erange(bp)[2] for bp in bps}
         f_by_kp = {kp: self.frange(kp)[2] for kp in kps}
         e_all = sorted({l for bp in bps for l in e_by_bp[bp]})
-        two_level = len(bps) > 1 or (bps and wbra(bps[0]))
-        # a single bra part with a weight folds the weight into the ket sum
-        fold_bra = (not two_level) or False
+        # several bra parts: contract the ket sum first (tmp), then weight by
+        # the bra part; one bra part: its weight folds into the ket weight
+        two_level = len(bps) > 1
         M = max(max(e_by_bp[bp]) for bp in bps) + max(max(f_by_kp[kp]) for kp in kps)
         targets = []
         seen = set()
@@ -322,6 +322,7 @@ class Gen:
                 bp = bps[0]
                 if kp not in kps_by_bp[bp]:
                     continue
+                wk = "*".join(x for x in (wbra(bp), wk) if x)
                 for le in e_by_bp[bp]:
                     for lf in f_by_kp[kp]:
                         for e in cart(le):
@@ -536,36 +537,40 @@ class Gen:
         W("  }")
         W("}")
 
+    def combo_M(self, combo):
+        bp, kp = combo
+        return max(self.erange(bp)[2]) + max(self.frange(kp)[2])
+
     def kernel_block(self, W, ops_prim, ops_bra, ops_ctr):
-        """One thread per quartet of the chunk [a.q0, a.q1); the L-shell part
-        combinations are evaluated one after another and every integral is
-        stored to a.eblk (quartet-major, oriented layout); digestion follows
-        in k_digest (heavy classes)."""
+        """One kernel per L-shell part combination (so each gets the register
+        budget of that combination alone): one thread per quartet of the
+        chunk [a.q0, a.q1), its sub-block stored to a.eblk (quartet-major,
+        oriented layout); digestion follows in k_digest (heavy classes)."""
         cls = self.cls
-        M = self.M
         nA, nB, nC, nD = self.dims()
-        W("template <typename R>")
-        W("__global__ void __launch_bounds__(HFX_NT) eri_%s_blk(const ClassArgs a) {" % cls)
-        W("  extern __shared__ char hfx_smem[];")
-        W("  R* btab = reinterpret_cast<R*>(hfx_smem);")
-        W("  constexpr int NB_COL = %d + BoysTerms<R>::K;" % M)
-        W("  stage_boys<R>(btab, a, NB_COL);")
-        W("  const int qend = min(a.q1, *a.count);")
-        W("  for (int iq = a.q0 + blockIdx.x * blockDim.x + threadIdx.x; iq < qend; iq += gridDim.x * blockDim.x) {")
-        W("    const int2 bk = a.list[iq];")
-        W("    double* out = a.eblk + (size_t)(iq - a.q0) * %d;" % (nA * nB * nC * nD))
-        self.emit_quartet_setup(W)
         n = 0
         for gi, combos in enumerate(self.group_plan(True)):
-            W("    {")
+            M = self.combo_M(combos[0])
+            W("template <typename R>")
+            W("__global__ void __launch_bounds__(HFX_NT) eri_%s_c%d(const ClassArgs a) {" % (cls, gi))
+            W("  extern __shared__ char hfx_smem[];")
+            W("  R* btab = reinterpret_cast<R*>(hfx_smem);")
+            W("  constexpr int NB_COL = %d + BoysTerms<R>::K;" % M)
+            W("  stage_boys<R>(btab, a, NB_COL);")
+            W("  const int qend = min(a.q1, *a.count);")
+            W("  for (int iq = a.q0 + blockIdx.x * blockDim.x + threadIdx.x; iq < qend; iq += gridDim.x * blockDim.x) {")
+            W("    const int2 bk = a.list[iq];")
+            W("    double* out = a.eblk + (size_t)(iq - a.q0) * %d;" % (nA * nB * nC * nD))
+            self.emit_quartet_setup(W)
             el = self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%d" % gi)
             for (ia, ib, ic, id_), nm in sorted(el.items()):
                 W("    out[%d] = (double)%s;" % (((ia * nB + ib) * nC + ic) * nD + id_, nm))
             n += len(el)
-            W("    }")
+            W("  }")
+            W("}")
+            W("template __global__ void eri_%s_c%d<double>(const ClassArgs);" % (cls, gi))
+            W("template __global__ void eri_%s_c%d<float>(const ClassArgs);" % (cls, gi))
         assert n == nA * nB * nC * nD
-        W("  }")
-        W("}")
 
     def kernel(self, split):
         L = []
@@ -578,8 +583,6 @@ class Gen:
         W("namespace hfx {")
         if split:
             self.kernel_block(W, ops_prim, ops_bra, ops_ctr)
-            W("template __global__ void eri_%s_blk<double>(const ClassArgs);" % self.cls)
-            W("template __global__ void eri_%s_blk<float>(const ClassArgs);" % self.cls)
             nA, nB, nC, nD = self.dims()
             # digestion in k_digest: 6 contractions of every element + block max
             ops_ctr.n += 13 * nA * nB * nC * nD
@@ -639,6 +642,7 @@ def main(outdir):
         info["flops_prim_fp32"] = info["flops_prim_vrr"] + (info["combos"] if split else 1) * boys_flops(
             info["M"], BOYS_TERMS["float"])
         info["dims"] = g.dims()
+        info["combo_M"] = [g.combo_M(grp[0]) for grp in g.group_plan(True)] if split else []
         info["swap_bra"] = g.swap_bra
         info["swap_ket"] = g.swap_ket
         table[cls] = info
@@ -647,24 +651,32 @@ def main(outdir):
          "#pragma once", "#include \"hfx_device.h\"", "namespace hfx {"]
     for c in names:
         if table[c]["split"]:
-            D.append("template <typename R> __global__ void eri_%s_blk(const ClassArgs);" % c)
+            for k in range(table[c]["combos"]):
+                D.append("template <typename R> __global__ void eri_%s_c%d(const ClassArgs);" % (c, k))
         else:
             D.append("template <typename R, bool JONLY> __global__ void eri_%s(const ClassArgs);" % c)
     D.append("struct ClassKernels {")
     D.append("  const void* f64; const void* f64j; const void* f32; int M; int all_prims;")
     D.append("  double flops_prim, flops_prim_fp32, flops_braprim, flops_quartet;")
     D.append("  int split; int dims[4]; int swap_bra, swap_ket;")
+    D.append("  int ncombo; int combo_M[16]; const void* c64[16]; const void* c32[16];")
     D.append("};")
     D.append("static const ClassKernels kClassKernels[%d] = {" % len(names))
     for c in names:
         t = table[c]
         if t["split"]:
-            fns = "(const void*)&eri_%s_blk<double>, (const void*)&eri_%s_blk<double>, (const void*)&eri_%s_blk<float>" % (c, c, c)
+            fns = "nullptr, nullptr, nullptr"
+            nc = t["combos"]
+            cm = "{" + ", ".join(str(x) for x in t["combo_M"]) + "}"
+            c64 = "{" + ", ".join("(const void*)&eri_%s_c%d<double>" % (c, k) for k in range(nc)) + "}"
+            c32 = "{" + ", ".join("(const void*)&eri_%s_c%d<float>" % (c, k) for k in range(nc)) + "}"
         else:
             fns = "(const void*)&eri_%s<double, false>, (const void*)&eri_%s<double, true>, (const void*)&eri_%s<float, false>" % (c, c, c)
-        D.append("  {%s, %d, %d, %d, %d, %d, %d, %d, {%d, %d, %d, %d}, %d, %d}," % (
+            nc, cm, c64, c32 = 0, "{0}", "{nullptr}", "{nullptr}"
+        D.append("  {%s, %d, %d, %d, %d, %d, %d, %d, {%d, %d, %d, %d}, %d, %d, %d, %s, %s, %s}," % (
             fns, t["M"], int(t["all_prims"]), t["flops_prim"], t["flops_prim_fp32"], t["flops_braprim"],
-            t["flops_quartet"], int(t["split"]), *t["dims"], int(t["swap_bra"]), int(t["swap_ket"])))
+            t["flops_quartet"], int(t["split"]), *t["dims"], int(t["swap_bra"]), int(t["swap_ket"]),
+            nc, cm, c64, c32))
     D.append("};")
     D.append("}  // namespace hfx")
     path = os.path.join(outdir, "eri_dispatch.h")

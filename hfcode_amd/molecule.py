@@ -136,6 +136,23 @@ def normalize(zetas, l, coefs):
     return [x / s for x in g]
 
 
+def atomic_density_guess(mol, basis):
+    """Superposition of the basis file's atomic density blocks, the guess
+    direct_hartreefock.f90:50-64 builds when USEDENSITYSUPGUESS is set."""
+    dens = basis.get("density")
+    if not dens:
+        raise ValueError("basis %s carries no atomic densities" % basis["name"])
+    blocks = [np.array(dens[l]) for l in mol.labels]
+    n = sum(b.shape[0] for b in blocks)
+    P = np.zeros((n, n))
+    k = 0
+    for b in blocks:
+        m = b.shape[0]
+        P[k:k + m, k:k + m] = b
+        k += m
+    return P
+
+
 class ShellTable:
     """Flat shell arrays in the layout the C-ABI takes (hfx_mol_desc).
 

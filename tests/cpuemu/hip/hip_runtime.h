@@ -35,6 +35,10 @@ struct int2 {
   int x, y;
 };
 inline int2 make_int2(int a, int b) { return int2{a, b}; }
+struct int4 {
+  int x, y, z, w;
+};
+inline int4 make_int4(int a, int b, int c, int d) { return int4{a, b, c, d}; }
 
 struct hfx_emu_idx {
   unsigned x, y, z;
