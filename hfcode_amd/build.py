@@ -38,9 +38,9 @@ def sources():
 
 def compile_one(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    deps = [src, os.path.join(CSRC, "hfx_device.h")]
+    deps = [src, os.path.join(CSRC, "hfx_device.h"), os.path.join(CSRC, "hfx_kernels.h")]
     if os.path.basename(src).startswith("hfx_"):
-        deps += [os.path.join(CSRC, "hfx_kernels.h"), os.path.join(GEN, "eri_dispatch.h"),
+        deps += [os.path.join(GEN, "eri_dispatch.h"),
                  os.path.join(ROOT, "include", "hfx.h")]
     if os.path.exists(obj) and all(os.path.getmtime(obj) >= os.path.getmtime(d) for d in deps if os.path.exists(d)):
         return obj, None

@@ -53,12 +53,24 @@ struct ClassArgs {
   const float* boys32;
   double* blocks;          // optional: dump of every block (625 per quartet)
   double* schw_out;        // optional: sqrt(max|(ab|ab)|) per bra pair
-  double* eblk;            // block kernels: integrals of quartets [q0, q1), quartet-major
+  double* eblk;            // block kernels: integrals of quartets [q0, q1), tiles of 64 quartets, element-major
+  double* vmax;            // block kernels: max|e| of each combination's part, [combination][quartet - q0]
   double thr;
   int nb;
   int diag;                // 0 off-diagonal build, 1 bra==ket build
   int q0, q1;              // block kernels: chunk of the quartet list
+  int vstride;             // block kernels: row length of vmax
 };
+
+// Compiler barrier between slices of an unrolled load sequence: keeps the
+// scheduler from hoisting every load of a block into registers at once.
+#define HFX_LOAD_FENCE() asm volatile("" ::: "memory")
+// Materialises a running accumulator at this point: its updates cannot be
+// sunk past it (into the conditional flush at the end of a pass), which
+// would keep every loaded element alive until then.
+#ifndef HFX_PIN
+#define HFX_PIN(x) asm volatile("" : "+v"(x))
+#endif
 
 // Wave-level barrier: LDS written by one lane is visible to the others.
 __device__ inline void wave_sync() {

@@ -111,7 +111,8 @@ struct hfx_ctx {
   int debug_blocks = 0;
   int profile = 0;
   double* d_eblk = nullptr;     // integral blocks of the block class kernels
-  long long eblk_quartets = 0;  // chunk size (quartets) the buffer holds
+  double* d_vmax = nullptr;     // their partial max|e|, kMaxCombo rows of kMaxChunk
+  size_t max_shm = 0;           // LDS bytes a workgroup may use
   int class_ran[2][HFX_NCLASSES] = {};
   hipEvent_t cev[2][HFX_NCLASSES][2] = {};
   int last_kind = 0;  // 0 none, 1 non-diag, 2 diag
@@ -127,6 +128,10 @@ struct hfx_ctx {
   double* d_F = nullptr;
   double* d_X = nullptr;
   double* d_maxdm = nullptr;
+  unsigned long long* d_pmax = nullptr;        // max over maxdm (bits of the double)
+  unsigned long long* d_masks = nullptr;       // screening survivor bits of one class
+  size_t masks_words = 0;
+  int act_sorted[HFX_NPAIRTYPES] = {0};         // active Schwarz values non-increasing
   double* d_schw_out = nullptr;   // per global pair
   int* d_act[HFX_NPAIRTYPES] = {nullptr};        // active global pair indices (rank order)
   double* d_act_schw[HFX_NPAIRTYPES] = {nullptr};
@@ -171,6 +176,9 @@ int upload_active(hfx_ctx* c) {
       pr.rank = k;
       info[k] = make_int4(pr.sha, pr.shb, pr.np, pr.npall);
     }
+    c->act_sorted[t] = 1;
+    for (int k = 1; k < c->last[t]; ++k)
+      if (asc[k] > asc[k - 1]) c->act_sorted[t] = 0;
     if (c->d_act[t]) { hipFree(c->d_act[t]); c->d_act[t] = nullptr; }
     if (c->d_act_schw[t]) { hipFree(c->d_act_schw[t]); c->d_act_schw[t] = nullptr; }
     if (c->d_act_info[t]) { hipFree(c->d_act_info[t]); c->d_act_info[t] = nullptr; }
@@ -184,12 +192,19 @@ int upload_active(hfx_ctx* c) {
   }
   HIPOK(hipMemcpy(c->d_pairs, c->pairs.data(), c->pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice));
   // class plans and their work lists
+  size_t need_masks = 0;
   for (auto& cp : c->classes) {
     if (cp.list64) { hipFree(cp.list64); cp.list64 = nullptr; }
     if (cp.list32) { hipFree(cp.list32); cp.list32 = nullptr; }
     const long long nb = c->last[cp.t1], nk = c->last[cp.t2];
     cp.cap = cp.same ? nb * (nb - 1) / 2 : nb * nk;
     if (cp.cap > 0 && dalloc(&cp.list64, (size_t)cp.cap)) return HFX_E_NOMEM;
+    need_masks = std::max(need_masks, (size_t)nb * (size_t)((nk + HFX_WAVE - 1) / HFX_WAVE) * 2);
+  }
+  if (need_masks > c->masks_words) {
+    if (c->d_masks) { hipFree(c->d_masks); c->d_masks = nullptr; }
+    if (dalloc(&c->d_masks, need_masks)) return HFX_E_NOMEM;
+    c->masks_words = need_masks;
   }
   c->have_schwarz = true;
   return HFX_OK;
@@ -229,23 +244,36 @@ int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   return HFX_OK;
 }
 
-const long long kEblkChunk = 1 << 18;  // quartets per block-kernel chunk (1.3 GB at 625 doubles)
+const long long kEblkDoubles = (1ll << 18) * 625;  // integral buffer of the block classes (1.3 GB)
+const long long kMaxChunk = 1 << 20;                 // quartets per chunk, at most
+const int kMaxCombo = 16;                            // combination kernels per class, at most
 
-// Heavy classes: block kernel (integrals to HBM) + wave-per-quartet digestion,
-// chunk by chunk over quartets [0, n).
+// Heavy classes: combination kernels (integrals to HBM, tiles of 64
+// quartets) + the class's digestion kernel, chunk by chunk over quartets
+// [0, n).  The digestion walks contiguous ranges of the (bra-contiguous)
+// list and, where the workgroup's LDS holds the rows of X that a bra pair's
+// quartets update (rows of shells A and B: J_ab, K_ac, K_ad, K_bc, K_bd),
+// accumulates them there and flushes them with row-coalesced atomics.
 int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long long n, double* dump) {
   const ClassKernels& k = kClassKernels[cls];
   if (n <= 0) return HFX_OK;
   if (!c->d_eblk) {
-    if (dalloc(&c->d_eblk, (size_t)kEblkChunk * 625)) return HFX_E_NOMEM;
-    c->eblk_quartets = kEblkChunk;
+    if (dalloc(&c->d_eblk, (size_t)kEblkDoubles)) return HFX_E_NOMEM;
+    if (dalloc(&c->d_vmax, (size_t)kMaxChunk * kMaxCombo)) return HFX_E_NOMEM;
   }
+  const long long ne = (long long)k.dims[0] * k.dims[1] * k.dims[2] * k.dims[3];
+  const long long chunk = std::min(kMaxChunk, (kEblkDoubles / ne) / HFX_NT * HFX_NT);
   const int K = prec == 1 ? 5 : 8;
+  // LDS rows of X for the digestion (off for the diagonal builds: every bra once)
+  const size_t rows_shm = (size_t)(k.dims[0] + k.dims[1]) * c->nb * sizeof(double);
+  const bool rows = !a.diag && rows_shm <= c->max_shm;
   if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
-  for (long long q0 = 0; q0 < n; q0 += c->eblk_quartets) {
-    const long long q1 = std::min(n, q0 + c->eblk_quartets);
+  for (long long q0 = 0; q0 < n; q0 += chunk) {
+    const long long q1 = std::min(n, q0 + chunk);
     ClassArgs b = a;
     b.eblk = c->d_eblk;
+    b.vmax = c->d_vmax;
+    b.vstride = (int)chunk;
     b.q0 = (int)q0;
     b.q1 = (int)q1;
     void* args[] = {&b};
@@ -259,6 +287,8 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     g.list = a.list;
     g.count = a.count;
     g.eblk = c->d_eblk;
+    g.vmax = c->d_vmax;
+    g.vstride = (int)chunk;
     g.pairs = c->d_pairs;
     g.P = a.P;
     g.X = a.X;
@@ -270,11 +300,15 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     g.jonly = jonly;
     g.q0 = (int)q0;
     g.q1 = (int)q1;
+    g.lds_rows = rows ? 1 : 0;
     for (int i = 0; i < 4; ++i) g.dims[i] = k.dims[i];
     g.swap_bra = k.swap_bra;
     g.swap_ket = k.swap_ket;
-    hipLaunchKernelGGL(k_digest, dim3(grid_for(q1 - q0, 4, c->ncu * 16)), dim3(256), 0, c->stream, g);
-    HIPOK(hipGetLastError());
+    void* gargs[] = {&g};
+    // with LDS rows: two workgroups per CU, each a contiguous range of steps
+    const int grid = rows ? grid_for(q1 - q0, HFX_NT, c->ncu * 2) : grid_for(q1 - q0, HFX_NT, c->ncu * 8);
+    HIPOK(hipLaunchKernel(jonly ? k.digj : k.dig, dim3(grid), dim3(HFX_NT), gargs, rows ? rows_shm : 0,
+                          c->stream));
   }
   if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));
   c->class_ran[prec][cls] = 1;
@@ -287,13 +321,14 @@ int zero_build_state(hfx_ctx* c) {
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)nx, 256, 2048)), dim3(256), 0, c->stream, c->d_X, nx, 0.0);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, c->stream, c->d_counts, 2 * HFX_NCLASSES, 0);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, c->stream, c->d_stats, 2 * HFX_NCLASSES * 3);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->d_pmax, 1);
   HIPOK(hipGetLastError());
   return HFX_OK;
 }
 
 int launch_maxdm(hfx_ctx* c, const double* dP) {
   const long long n = (long long)c->nsh * c->nsh;
-  hipLaunchKernelGGL(k_maxdm, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->stream, dP, c->nb, c->d_shells, c->nsh, c->d_maxdm);
+  hipLaunchKernelGGL(k_maxdm, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->stream, dP, c->nb, c->d_shells, c->nsh, c->d_maxdm, c->d_pmax);
   HIPOK(hipGetLastError());
   return HFX_OK;
 }
@@ -349,8 +384,12 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode) {
     s.jonly = jonly;
     s.nsh = c->nsh;
     s.all_prims = kClassKernels[cp.cls].all_prims;
-    const long long n = (long long)s.nbra * s.nket;
-    hipLaunchKernelGGL(k_screen, dim3(grid_for(n, 256, c->ncu * 16)), dim3(256), 0, c->stream, s);
+    s.pmax = c->d_pmax;
+    s.masks = c->d_masks;
+    s.nchunk = (s.nket + HFX_WAVE - 1) / HFX_WAVE;
+    s.ket_sorted = c->act_sorted[cp.t2];
+    // one wave per bra position
+    hipLaunchKernelGGL(k_screen, dim3(grid_for(s.nbra, 256 / HFX_WAVE, c->ncu * 16)), dim3(256), 0, c->stream, s);
     HIPOK(hipGetLastError());
   }
   // surviving-quartet counts of the block classes decide their chunking
@@ -608,6 +647,7 @@ int hfx_ctx_create(const hfx_mol_desc* d, hfx_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, c->dev) != hipSuccess) return fail(HFX_E_HIP);
   c->ncu = prop.multiProcessorCount;
+  c->max_shm = prop.sharedMemPerBlock;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(HFX_E_HIP);
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(HFX_E_HIP);
   std::vector<ShellRec> sr(c->nsh);
@@ -619,7 +659,7 @@ int hfx_ctx_create(const hfx_mol_desc* d, hfx_ctx** out) {
   if ((r = dalloc(&c->d_pairs, c->pairs.size())) || (r = dalloc(&c->d_prims, c->prims.size())) ||
       (r = dalloc(&c->d_shells, sr.size())) || (r = dalloc(&c->d_boys64, b64.size())) ||
       (r = dalloc(&c->d_boys32, b32.size())) || (r = dalloc(&c->d_P, nb2)) || (r = dalloc(&c->d_F, nb2)) ||
-      (r = dalloc(&c->d_X, nb2)) || (r = dalloc(&c->d_maxdm, (size_t)c->nsh * c->nsh)) ||
+      (r = dalloc(&c->d_X, nb2)) || (r = dalloc(&c->d_maxdm, (size_t)c->nsh * c->nsh)) || (r = dalloc(&c->d_pmax, 1)) ||
       (r = dalloc(&c->d_schw_out, (size_t)tot)) || (r = dalloc(&c->d_counts, 2 * HFX_NCLASSES)) ||
       (r = dalloc(&c->d_stats, 2 * HFX_NCLASSES * 3)))
     return fail(r);
@@ -657,7 +697,7 @@ void hfx_ctx_destroy(hfx_ctx* c) {
   }
   hipFree(c->d_pairs); hipFree(c->d_prims); hipFree(c->d_shells); hipFree(c->d_boys64); hipFree(c->d_boys32);
   hipFree(c->d_P); hipFree(c->d_F); hipFree(c->d_X); hipFree(c->d_maxdm); hipFree(c->d_schw_out);
-  hipFree(c->d_counts); hipFree(c->d_stats); hipFree(c->d_eblk);
+  hipFree(c->d_counts); hipFree(c->d_stats); hipFree(c->d_eblk); hipFree(c->d_vmax); hipFree(c->d_pmax); hipFree(c->d_masks);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   for (int p = 0; p < 2; ++p)

@@ -19,9 +19,13 @@ struct ScreenArgs {
   int* count32;
   unsigned long long* stats64; // [quartets, primitive quartets, bra primitives]
   unsigned long long* stats32;
+  const unsigned long long* pmax;  // max over maxdm, bits of a non-negative double
+  unsigned long long* masks;   // survivor bits: [bra][chunk of HFX_WAVE kets][list 64, list 32]
   double thr;
   double split;                // > 0: bound < split -> FP32 list
   int nbra, nket, same, jonly, nsh, all_prims;
+  int nchunk;                  // chunks per bra row of masks
+  int ket_sorted;              // schw_ket is non-increasing (the early stop applies)
 };
 
 struct DiagScreenArgs {
@@ -38,7 +42,9 @@ struct DiagScreenArgs {
 struct DigestArgs {
   const int2* list;            // quartet list of the class
   const int* count;            // its length
-  const double* eblk;          // integrals of quartets [q0, q1), quartet-major, oriented layout
+  const double* eblk;          // integrals of quartets [q0, q1): tiles of 64 quartets, element-major, oriented layout
+  const double* vmax;          // partial max|e| per combination kernel, [combination][quartet - q0]
+  int vstride;
   const PairRec* pairs;
   const double* P;
   double* X;
@@ -46,15 +52,16 @@ struct DigestArgs {
   double* schw_out;            // optional: sqrt(max|e|) per bra pair (initial diagonal build)
   double thr;
   int nb, diag, jonly, q0, q1;
+  int lds_rows;                // accumulate rows A and B of X in LDS ((nA + nB) x nb doubles)
   int dims[4];                 // components of the oriented shells A, B, C, D
   int swap_bra, swap_ket;      // oriented A is the pair's second shell (bra / ket)
 };
 
-__global__ void k_digest(const DigestArgs d);
 __global__ void k_fill_f64(double* x, size_t n, double v);
 __global__ void k_fill_i32(int* x, int n, int v);
 __global__ void k_fill_u64(unsigned long long* x, int n);
-__global__ void k_maxdm(const double* P, int nb, const ShellRec* sh, int nsh, double* maxdm);
+__global__ void k_maxdm(const double* P, int nb, const ShellRec* sh, int nsh, double* maxdm,
+                        unsigned long long* pmax);
 __global__ void k_screen(const ScreenArgs s);
 __global__ void k_screen_diag(const DiagScreenArgs s);
 __global__ void k_add_sym(const double* X, double* F, int nb);

@@ -560,17 +560,227 @@ class Gen:
             W("  const int qend = min(a.q1, *a.count);")
             W("  for (int iq = a.q0 + blockIdx.x * blockDim.x + threadIdx.x; iq < qend; iq += gridDim.x * blockDim.x) {")
             W("    const int2 bk = a.list[iq];")
-            W("    double* out = a.eblk + (size_t)(iq - a.q0) * %d;" % (nA * nB * nC * nD))
+            W("    double* out = a.eblk + (size_t)((iq - a.q0) >> 6) * %d + ((iq - a.q0) & 63);" % (nA * nB * nC * nD * 64))
             self.emit_quartet_setup(W)
             el = self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%d" % gi)
+            # stored value: the diagonal builds drop |e| <= 1e-10 (fock_build_diag.f90:22);
+            # the partial max|e| of the raw values feeds the block test
+            W("    const bool ecut = a.diag != 0;")
+            W("    double vm = 0;")
             for (ia, ib, ic, id_), nm in sorted(el.items()):
-                W("    out[%d] = (double)%s;" % (((ia * nB + ib) * nC + ic) * nD + id_, nm))
+                W("    { double o = (double)%s; vm = fmax(vm, fabs(o)); if (ecut && fabs(o) <= HFX_INTTHRESH) o = 0;"
+                  " out[%d] = o; }" % (nm, (((ia * nB + ib) * nC + ic) * nD + id_) * 64))
+            W("    a.vmax[(size_t)%d * a.vstride + (iq - a.q0)] = vm;" % gi)
             n += len(el)
             W("  }")
             W("}")
             W("template __global__ void eri_%s_c%d<double>(const ClassArgs);" % (cls, gi))
             W("template __global__ void eri_%s_c%d<float>(const ClassArgs);" % (cls, gi))
         assert n == nA * nB * nC * nD
+
+    def kernel_digest(self, W, ops_ctr):
+        """Digestion of the stored blocks of one block class: one thread per
+        quartet (tile-major blocks, so every element load is one coalesced
+        wave access).  The block test (max|e| < thr skips the quartet,
+        fock_build_non_diag.f90:69) uses the partial maxima the combination
+        kernels stored.  Three passes over the block, each flushing an
+        accumulator as soon as it is complete:
+          J:  X_ab += 2w sum_cd e P_cd (per (a,b)),  X_cd += 2w sum_ab e P_ab;
+          K1: X_ac -= w/2 sum_bd e P_bd, X_ad -= w/2 sum_bc e P_bc (per a);
+          K2: X_bc -= w/2 sum_ad e P_ad, X_bd -= w/2 sum_ac e P_ac (per b).
+        Rows of A and B: with d.lds_rows the workgroup walks a contiguous
+        range of the bra-contiguous list 256 quartets at a time and keeps the
+        rows of X of one bra pair (the bra of the last quartet of the step)
+        in LDS; quartets of that bra add there (X_ab summed over the wave
+        first), the rows go to X with row-coalesced atomics when the bra
+        changes and at the end.  Other quartets, and X_cd, add to X directly.
+        A compiler barrier after each slice of loads keeps at most one slice
+        of the block in registers."""
+        cls = self.cls
+        nA, nB, nC, nD = self.dims()
+        ne = nA * nB * nC * nD
+        ncombo = len(self.group_plan(True))
+        oa = "b" if self.swap_bra else "a"
+        ob = "a" if self.swap_bra else "b"
+        oc = "b" if self.swap_ket else "a"
+        od = "a" if self.swap_ket else "b"
+
+        def lin(ia, ib, ic, id_):
+            return (((ia * nB + ib) * nC + ic) * nD + id_) * 64
+
+        def pload(name, ao1, n1, ao2, n2):
+            for i in range(n1):
+                for j in range(n2):
+                    W("      const double %s_%d_%d = d.P[(size_t)(%s + %d) * nbf + %s + %d];" % (name, i, j, ao1, i, ao2, j))
+
+        def zero(name, n):
+            W("      " + " ".join("double %s_%d = 0;" % (name, i) for i in range(n)))
+
+        def pin(name, n):
+            W("      " + " ".join("HFX_PIN(%s_%d);" % (name, i) for i in range(n)) + " HFX_LOAD_FENCE();")
+
+        def flush_row(name, n, ao1, i, row, ao2, wname):
+            """accumulator row `name` (n columns from ao2) of X row ao1 + i;
+            LDS row `row` when the quartet's bra is the resident one"""
+            W("      if (inl) {")
+            for j in range(n):
+                W("        atomicAdd(&srow[%d * nbf + %s + %d], %s * %s_%d);" % (row, ao2, j, wname, name, j))
+            W("      } else if (act) {")
+            for j in range(n):
+                W("        atomicAdd(&X[(size_t)(%s + %d) * nbf + %s + %d], %s * %s_%d);" % (ao1, i, ao2, j, wname, name, j))
+            W("      }")
+
+        W("template <bool JONLY>")
+        W("__global__ void __launch_bounds__(HFX_NT) dig_%s(const DigestArgs d) {" % cls)
+        W("  extern __shared__ char hfx_smem[];")
+        W("  double* srow = reinterpret_cast<double*>(hfx_smem);   // [%d][nb]: rows A then rows B" % (nA + nB))
+        W("  const int qend = min(d.q1, *d.count);")
+        W("  if (qend <= d.q0) return;")
+        W("  const int nbf = d.nb;")
+        W("  const bool rows = d.lds_rows != 0;")
+        W("  const int lane = threadIdx.x & (HFX_WAVE - 1);")
+        W("  double* X = d.X;")
+        W("  const int nsteps = (qend - d.q0 + blockDim.x - 1) / blockDim.x;")
+        W("  int s0, s1;")
+        W("  if (rows) {   // contiguous range of steps per workgroup")
+        W("    const int per = (nsteps + gridDim.x - 1) / gridDim.x;")
+        W("    s0 = blockIdx.x * per; s1 = min(nsteps, s0 + per);")
+        W("    for (int e = threadIdx.x; e < %d * nbf; e += blockDim.x) srow[e] = 0.0;" % (nA + nB))
+        W("  } else {")
+        W("    s0 = blockIdx.x; s1 = nsteps;")
+        W("  }")
+        W("  const int sstride = rows ? 1 : gridDim.x;")
+        W("  int curb = -1, rA = 0, rB = 0;   // resident bra pair and its rows")
+        W("  auto flush = [&]() {")
+        W("    __syncthreads();")
+        W("    for (int e = threadIdx.x; e < %d * nbf; e += blockDim.x) {" % (nA + nB))
+        W("      const double v = srow[e];")
+        W("      if (v != 0.0) {")
+        W("        const int r = e / nbf, col = e - r * nbf;")
+        W("        const int row = r < %d ? rA + r : rB + (r - %d);" % (nA, nA))
+        W("        atomicAdd(&X[(size_t)row * nbf + col], v);")
+        W("        srow[e] = 0.0;")
+        W("      }")
+        W("    }")
+        W("    __syncthreads();")
+        W("  };")
+        W("  for (int st = s0; st < s1; st += sstride) {")
+        W("    const int it = d.q0 + st * blockDim.x + (threadIdx.x - lane);   // first quartet of the wave")
+        W("    if (rows) {")
+        W("      const int tb = d.list[min(d.q0 + (st + 1) * (int)blockDim.x, qend) - 1].x;")
+        W("      if (tb != curb) {")
+        W("        if (curb >= 0) flush();")
+        W("        curb = tb;")
+        W("        rA = d.pairs[tb].ao%s; rB = d.pairs[tb].ao%s;" % (oa, ob))
+        W("      }")
+        W("    }")
+        W("    if (it >= qend) continue;")
+        W("    const bool valid = it + lane < qend;")
+        W("    const int iq = valid ? it + lane : qend - 1;")
+        W("    const int slot = iq - d.q0;")
+        W("    const int2 bk = d.list[iq];")
+        W("    double vmax = d.vmax[slot];")
+        for c in range(1, ncombo):
+            W("    vmax = fmax(vmax, d.vmax[(size_t)%d * d.vstride + slot]);" % c)
+        W("    if (d.schw_out != nullptr && valid) d.schw_out[bk.x] = sqrt(vmax);")
+        W("    const bool act = valid && (d.diag || vmax >= d.thr);")
+        W("    const double* blk = d.eblk + (size_t)(slot >> 6) * %d + (slot & 63);" % (ne * 64))
+        W("    if (d.dump != nullptr && valid) {")
+        W("      #pragma unroll 1")
+        W("      for (int e = 0; e < %d; ++e) {" % ne)
+        W("        const int id = e %% %d, ic = (e / %d) %% %d, ib = (e / %d) %% %d, ia = e / %d;" % (
+            nD, nD, nC, nD * nC, nB, nD * nC * nB))
+        i1, i2 = ("ib", "ia") if self.swap_bra else ("ia", "ib")
+        i3, i4 = ("id", "ic") if self.swap_ket else ("ic", "id")
+        m1, m2 = (nB, nA) if self.swap_bra else (nA, nB)
+        m3 = nD if self.swap_ket else nC
+        W("        d.dump[(size_t)iq * 625 + %s + %d * (%s + %d * (%s + %d * %s))] = blk[(size_t)e * 64];" % (
+            i1, m1, i2, m2, i3, m3, i4))
+        W("      }")
+        W("    }")
+        W("    if (__ballot(act) == 0) continue;")
+        W("    const bool inl = rows && act && bk.x == curb;   // rows A, B resident in LDS")
+        W("    const PairRec& pb = d.pairs[bk.x];")
+        W("    const PairRec& pk = d.pairs[bk.y];")
+        W("    const int shA = pb.sh%s, shB = pb.sh%s, shC = pk.sh%s, shD = pk.sh%s;" % (oa, ob, oc, od))
+        W("    const int aoA = pb.ao%s, aoB = pb.ao%s, aoC = pk.ao%s, aoD = pk.ao%s;" % (oa, ob, oc, od))
+        W("    const double w = act ? ((shA == shB) ? 0.5 : 1.0) * ((shC == shD) ? 0.5 : 1.0) * (d.diag ? 0.5 : 1.0) : 0.0;")
+        W("    const double wj = 2.0 * w, wk = -0.5 * w;")
+        # pass J (each pass re-derives the block pointer so that the element
+        # addresses are not shared, and kept live, across passes)
+        W("    {")
+        W("      const double* bp = blk; HFX_PIN(bp);")
+        pload("Pcd", "aoC", nC, "aoD", nD)
+        zero("Jcd", nC * nD)
+        for ia in range(nA):
+            for ib in range(nB):
+                W("      {")
+                W("      const double pab = d.P[(size_t)(aoA + %d) * nbf + aoB + %d];" % (ia, ib))
+                W("      double jab = 0;")
+                for ic in range(nC):
+                    for id_ in range(nD):
+                        W("      { const double v = bp[%d]; jab = fma(v, Pcd_%d_%d, jab); Jcd_%d = fma(v, pab, Jcd_%d); }" % (
+                            lin(ia, ib, ic, id_), ic, id_, ic * nD + id_, ic * nD + id_))
+                W("      jab *= wj;")
+                W("      if (rows) {")
+                W("        double r = inl ? jab : 0.0;")
+                W("        for (int o = HFX_WAVE / 2; o > 0; o >>= 1) r += __shfl_xor(r, o);")
+                W("        if (lane == 0 && r != 0.0) atomicAdd(&srow[%d * nbf + rB + %d], r);" % (ia, ib))
+                W("      }")
+                W("      if (act && !inl) atomicAdd(&X[(size_t)(aoA + %d) * nbf + aoB + %d], jab);" % (ia, ib))
+                pin("Jcd", nC * nD)
+                W("      }")
+        W("      if (act) {")
+        for ic in range(nC):
+            for id_ in range(nD):
+                W("        atomicAdd(&X[(size_t)(aoC + %d) * nbf + aoD + %d], wj * Jcd_%d);" % (ic, id_, ic * nD + id_))
+        W("      }")
+        W("    }")
+        W("    if constexpr (!JONLY) {")
+        # pass K1
+        W("    {")
+        W("      const double* bp = blk; HFX_PIN(bp);")
+        pload("Pbd", "aoB", nB, "aoD", nD)
+        pload("Pbc", "aoB", nB, "aoC", nC)
+        for ia in range(nA):
+            W("      {")
+            zero("Kac", nC)
+            zero("Kad", nD)
+            for ib in range(nB):
+                for ic in range(nC):
+                    for id_ in range(nD):
+                        W("      { const double v = bp[%d]; Kac_%d = fma(v, Pbd_%d_%d, Kac_%d); Kad_%d = fma(v, Pbc_%d_%d, Kad_%d); }" % (
+                            lin(ia, ib, ic, id_), ic, ib, id_, ic, id_, ib, ic, id_))
+                pin("Kac", nC); pin("Kad", nD)
+            flush_row("Kac", nC, "aoA", ia, ia, "aoC", "wk")
+            flush_row("Kad", nD, "aoA", ia, ia, "aoD", "wk")
+            W("      }")
+        W("    }")
+        # pass K2
+        W("    {")
+        W("      const double* bp = blk; HFX_PIN(bp);")
+        pload("Pad", "aoA", nA, "aoD", nD)
+        pload("Pac", "aoA", nA, "aoC", nC)
+        for ib in range(nB):
+            W("      {")
+            zero("Kbc", nC)
+            zero("Kbd", nD)
+            for ia in range(nA):
+                for ic in range(nC):
+                    for id_ in range(nD):
+                        W("      { const double v = bp[%d]; Kbc_%d = fma(v, Pad_%d_%d, Kbc_%d); Kbd_%d = fma(v, Pac_%d_%d, Kbd_%d); }" % (
+                            lin(ia, ib, ic, id_), ic, ia, id_, ic, id_, ia, ic, id_))
+                pin("Kbc", nC); pin("Kbd", nD)
+            flush_row("Kbc", nC, "aoB", ib, nA + ib, "aoC", "wk")
+            flush_row("Kbd", nD, "aoB", ib, nA + ib, "aoD", "wk")
+            W("      }")
+        W("    }")
+        W("    }")
+        W("  }")
+        W("  if (rows && curb >= 0) flush();")
+        W("}")
+        W("template __global__ void dig_%s<false>(const DigestArgs);" % cls)
+        W("template __global__ void dig_%s<true>(const DigestArgs);" % cls)
 
     def kernel(self, split):
         L = []
@@ -580,11 +790,13 @@ class Gen:
         W("// build centres: A=%s B=%s C=%s D=%s; Boys m <= %d; %s" % (
             self.A, self.B, self.C, self.D, self.M, "block + k_digest" if split else "fused digestion"))
         W("#include \"hfx_device.h\"")
+        if split:
+            W("#include \"hfx_kernels.h\"")
         W("namespace hfx {")
         if split:
             self.kernel_block(W, ops_prim, ops_bra, ops_ctr)
             nA, nB, nC, nD = self.dims()
-            # digestion in k_digest: 6 contractions of every element + block max
+            # digestion in dig_<class>: 6 contractions of every element + block max
             ops_ctr.n += 13 * nA * nB * nC * nD
         else:
             self.kernel_fused(W, ops_prim, ops_bra, ops_ctr)
@@ -602,6 +814,19 @@ class Gen:
             "all_prims": self.all_prims,
         }
         return "\n".join(L) + "\n", info
+
+    def digest_source(self):
+        """Translation unit of the class's digestion kernel (block classes)."""
+        L = []
+        W = L.append
+        W("// digestion of (%s|%s) blocks, generated by hfcode_amd/gen/gen_eri.py -- do not edit" % (
+            self.cls[:2], self.cls[2:]))
+        W("#include \"hfx_device.h\"")
+        W("#include \"hfx_kernels.h\"")
+        W("namespace hfx {")
+        self.kernel_digest(W, None)
+        W("}  // namespace hfx")
+        return "\n".join(L) + "\n"
 
     def fused_cost(self):
         """Doubles a fused kernel keeps live: accumulators + J/K partial sums + density blocks."""
@@ -633,6 +858,14 @@ def main(outdir):
         split = g.fused_cost() > SPLIT_COST
         src, info = g.kernel(split)
         path = os.path.join(outdir, "eri_%s.hip" % cls)
+        dpath = os.path.join(outdir, "dig_%s.hip" % cls)
+        if split:
+            dsrc = g.digest_source()
+            if not os.path.exists(dpath) or open(dpath).read() != dsrc:
+                with open(dpath, "w") as fh:
+                    fh.write(dsrc)
+        elif os.path.exists(dpath):
+            os.remove(dpath)
         old = open(path).read() if os.path.exists(path) else None
         if old != src:
             with open(path, "w") as fh:
@@ -648,11 +881,12 @@ def main(outdir):
         table[cls] = info
         names.append(cls)
     D = ["// generated by hfcode_amd/gen/gen_eri.py -- do not edit",
-         "#pragma once", "#include \"hfx_device.h\"", "namespace hfx {"]
+         "#pragma once", "#include \"hfx_device.h\"", "#include \"hfx_kernels.h\"", "namespace hfx {"]
     for c in names:
         if table[c]["split"]:
             for k in range(table[c]["combos"]):
                 D.append("template <typename R> __global__ void eri_%s_c%d(const ClassArgs);" % (c, k))
+            D.append("template <bool JONLY> __global__ void dig_%s(const DigestArgs);" % c)
         else:
             D.append("template <typename R, bool JONLY> __global__ void eri_%s(const ClassArgs);" % c)
     D.append("struct ClassKernels {")
@@ -660,6 +894,7 @@ def main(outdir):
     D.append("  double flops_prim, flops_prim_fp32, flops_braprim, flops_quartet;")
     D.append("  int split; int dims[4]; int swap_bra, swap_ket;")
     D.append("  int ncombo; int combo_M[16]; const void* c64[16]; const void* c32[16];")
+    D.append("  const void* dig; const void* digj;")
     D.append("};")
     D.append("static const ClassKernels kClassKernels[%d] = {" % len(names))
     for c in names:
@@ -670,13 +905,15 @@ def main(outdir):
             cm = "{" + ", ".join(str(x) for x in t["combo_M"]) + "}"
             c64 = "{" + ", ".join("(const void*)&eri_%s_c%d<double>" % (c, k) for k in range(nc)) + "}"
             c32 = "{" + ", ".join("(const void*)&eri_%s_c%d<float>" % (c, k) for k in range(nc)) + "}"
+            dg = "(const void*)&dig_%s<false>, (const void*)&dig_%s<true>" % (c, c)
         else:
             fns = "(const void*)&eri_%s<double, false>, (const void*)&eri_%s<double, true>, (const void*)&eri_%s<float, false>" % (c, c, c)
             nc, cm, c64, c32 = 0, "{0}", "{nullptr}", "{nullptr}"
-        D.append("  {%s, %d, %d, %d, %d, %d, %d, %d, {%d, %d, %d, %d}, %d, %d, %d, %s, %s, %s}," % (
+            dg = "nullptr, nullptr"
+        D.append("  {%s, %d, %d, %d, %d, %d, %d, %d, {%d, %d, %d, %d}, %d, %d, %d, %s, %s, %s, %s}," % (
             fns, t["M"], int(t["all_prims"]), t["flops_prim"], t["flops_prim_fp32"], t["flops_braprim"],
             t["flops_quartet"], int(t["split"]), *t["dims"], int(t["swap_bra"]), int(t["swap_ket"]),
-            nc, cm, c64, c32))
+            nc, cm, c64, c32, dg))
     D.append("};")
     D.append("}  // namespace hfx")
     path = os.path.join(outdir, "eri_dispatch.h")

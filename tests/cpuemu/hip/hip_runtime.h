@@ -17,6 +17,7 @@
 
 #ifndef HFX_WAVE
 #define HFX_WAVE 1
+#define HFX_PIN(x) asm volatile("" : "+m"(x))
 #endif
 
 #define __global__
@@ -57,6 +58,15 @@ inline int __ffsll(long long m) { return m ? __builtin_ffsll(m) : 0; }
 inline int __popcll(unsigned long long m) { return __builtin_popcountll(m); }
 template <typename T> inline T __shfl(T v, int) { return v; }
 template <typename T> inline T __shfl_xor(T v, int) { return v; }
+template <typename T> inline T __shfl_up(T v, unsigned) { return v; }
+template <typename T> inline T __shfl_down(T v, unsigned) { return v; }
+template <typename T> inline T atomicMax(T* p, T v) {
+  T old = *p;
+  if (v > old) *p = v;
+  return old;
+}
+inline long long __double_as_longlong(double x) { long long r; std::memcpy(&r, &x, 8); return r; }
+inline double __longlong_as_double(long long x) { double r; std::memcpy(&r, &x, 8); return r; }
 template <typename T> inline T atomicAdd(T* p, T v) {
   T old = *p;
   *p = old + v;
@@ -83,6 +93,7 @@ enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDevi
 #define hipEventDisableTiming 2
 struct hipDeviceProp_t {
   int multiProcessorCount;
+  size_t sharedMemPerBlock;
 };
 inline const char* hipGetErrorString(hipError_t) { return "emulated"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
@@ -107,6 +118,7 @@ inline hipError_t hipGetDevice(int* d) {
 }
 inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
   p->multiProcessorCount = 1;
+  p->sharedMemPerBlock = HFX_EMU_SMEM_BYTES;
   return hipSuccess;
 }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
