@@ -1,0 +1,182 @@
+"""Many-molecule jobs on one GPU: batched SCF solves and a serving loop.
+
+Each job is a full RHF solve (``hfcode_amd.scf.rhf``, the restatement of
+direct_hartreefock.f90:98-223) from the core-Hamiltonian guess, with its own
+Fock-build context (``FockBuilder``: device pair data, Schwarz tables, work
+lists and one HIP stream).  A pool of host worker threads runs the jobs; the
+library calls release the interpreter lock, so the jobs' Fock builds run
+concurrently on their own streams and fill the GPU that one small molecule
+alone would leave mostly idle.  Host linear algebra (S^-1/2, eigensolves,
+DIIS) runs with one BLAS thread per worker.
+
+Admission: a job is started only while the device memory its context would
+take (work-list capacity, integral buffers; estimated from its shell pairs
+before any allocation) fits in the budget next to the running jobs.
+"""
+import contextlib
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor, as_completed
+
+import numpy as np
+
+from . import fock, molecule as M, scf
+
+_basis_cache = {}
+_basis_lock = threading.Lock()
+
+
+def _basis(name):
+    with _basis_lock:
+        if name not in _basis_cache:
+            _basis_cache[name] = M.load_basis(name)
+        return _basis_cache[name]
+
+
+def memory_estimate(st):
+    """Device bytes a context for shell table `st` may take: the quartet
+    lists (capacity = candidate pair-pairs of every class, two lists with
+    mixed precision off: one), the integral buffer of the block classes
+    (bounded by the library at 1.3 GB), and nb^2 matrices."""
+    types = {}
+    for i in range(st.nsh):
+        for j in range(i + 1):
+            a, b = sorted((int(st.shtyp[i]), int(st.shtyp[j])), key=[1, 2, 3, 12].index)
+            types[(a, b)] = types.get((a, b), 0) + 1
+    npairs = list(types.values())
+    cap = 0
+    for x in range(len(npairs)):
+        for y in range(x, len(npairs)):
+            cap += npairs[x] * npairs[y] if x != y else npairs[x] * (npairs[x] - 1) // 2
+    eblk = min(cap * 625 * 8, (1 << 18) * 625 * 8)
+    return 8 * cap + eblk + 8 * 8 * st.nb * st.nb + (1 << 27)
+
+
+DIIS_START = float("inf")   # DIIS from the second iteration on
+
+
+def solve(mol, basis_name, maxiter=50, thr=scf.INTTHRESH, diis_start=DIIS_START):
+    """One RHF solve on the GPU; returns a result dict.
+
+    Guess: the superposition of the basis file's atomic densities where the
+    basis has them (STO-3G, 6-31G(d): the reference's USEDENSITYSUPGUESS,
+    direct_hartreefock.f90:50-64), else the core Hamiltonian (def2-SV(P),
+    whose basis file has none).  From the core guess, the reference's
+    schedule -- plain iterations until max|FPS - SPF| < 0.3, then DIIS
+    (direct_hartreefock.f90:145-150) -- oscillates for many chain molecules,
+    with or without a level shift (methanol 6-31G(d): E stays near
+    -91.5 Eh; with the same Fock builds the converged energy is -115.03 Eh);
+    the same DIIS (8 vectors, 1.05 damping, 1.5 cap) started at the second
+    iteration converges them, so the jobs run it that way."""
+    t0 = time.perf_counter()
+    basis = _basis(basis_name)
+    st = M.ShellTable(mol, basis)
+    fb = fock.FockBuilder(st)
+    try:
+        S, T, V = fb.onee(mol.znuc.astype(np.float64), mol.coords_bohr)
+        H = T + V
+        P0 = M.atomic_density_guess(mol, basis) if basis.get("density") else None
+        t1 = time.perf_counter()
+        res = scf.rhf(fb, S, H, mol.nocc, mol.nuclear_repulsion(), maxiter=maxiter, intthresh=thr,
+                      diis_start=diis_start, P0=P0)
+        t2 = time.perf_counter()
+    finally:
+        fb.close()
+    return dict(name=mol.title, natom=mol.natom, nb=st.nb, basis=basis_name, energy=res.energy,
+                iterations=res.iterations, converged=bool(res.converged), setup_s=t1 - t0, scf_s=t2 - t1)
+
+
+def _blas_one_thread():
+    """Context: one BLAS thread per calling thread (threadpoolctl), if available."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:     # BLAS keeps its own setting
+        return contextlib.nullcontext()
+    return threadpool_limits(1)
+
+
+def run_batch(mols, basis_name, workers=16, maxiter=50, progress=None):
+    """Solve every molecule of `mols`; returns (results in input order, wall
+    seconds).  `progress(done, total, elapsed_s)` is called as jobs finish."""
+    fock.lib()                      # load the library (and initialise the runtime) once
+    with _blas_one_thread():
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            futs = [ex.submit(solve, m, basis_name, maxiter) for m in mols]
+            if progress is not None:
+                done = 0
+                for _ in as_completed(futs):
+                    done += 1
+                    progress(done, len(futs), time.perf_counter() - t0)
+            out = [f.result() for f in futs]
+        wall = time.perf_counter() - t0
+    return out, wall
+
+
+class Server:
+    """Serving loop: requests (arrival time, molecule, basis) are admitted in
+    arrival order when a worker is free and the memory budget allows; the
+    latency of a request is completion time - arrival time."""
+
+    def __init__(self, workers=16, mem_budget=200 * (1 << 30), maxiter=50):
+        self.workers = workers
+        self.mem_budget = mem_budget
+        self.maxiter = maxiter
+        self._lock = threading.Condition()
+        self._inflight_mem = 0
+        self._inflight = 0
+        self._progress = None
+        self._total = 0
+
+    def _run_one(self, req, t_start, need, out):
+        arrival, mol, basis = req
+        try:
+            r = solve(mol, basis, self.maxiter)
+            r["error"] = None
+        except Exception as e:          # a failed job is reported, the loop goes on
+            r = dict(name=mol.title, natom=mol.natom, basis=basis, error=repr(e), converged=False)
+        done = time.perf_counter() - t_start
+        r.update(arrival_s=arrival, done_s=done, latency_s=done - arrival)
+        with self._lock:
+            self._inflight_mem -= need
+            self._inflight -= 1
+            out.append(r)
+            if self._progress is not None:
+                self._progress(len(out), self._total, done)
+            self._lock.notify_all()
+
+    def run(self, requests, progress=None):
+        """Replay `requests` in real time; returns (results, wall seconds).
+        `progress(done, total, elapsed_s)` is called as requests finish."""
+        fock.lib()
+        out = []
+        threads = []
+        self._progress, self._total = progress, len(requests)
+        with _blas_one_thread():
+            t_start = time.perf_counter()
+            for req in requests:
+                arrival, mol, basis = req
+                now = time.perf_counter() - t_start
+                if arrival > now:
+                    time.sleep(arrival - now)
+                need = memory_estimate(M.ShellTable(mol, _basis(basis)))
+                with self._lock:
+                    while self._inflight >= self.workers or (
+                            self._inflight > 0 and self._inflight_mem + need > self.mem_budget):
+                        self._lock.wait()
+                    self._inflight += 1
+                    self._inflight_mem += need
+                th = threading.Thread(target=self._run_one, args=(req, t_start, need, out), daemon=True)
+                th.start()
+                threads.append(th)
+            for th in threads:
+                th.join()
+            wall = time.perf_counter() - t_start
+        out.sort(key=lambda r: r["arrival_s"])
+        return out, wall
+
+
+def latency_summary(results):
+    lat = np.array([r["latency_s"] for r in results])
+    return dict(n=len(lat), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                mean_s=float(lat.mean()), max_s=float(lat.max()))

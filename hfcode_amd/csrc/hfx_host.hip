@@ -111,7 +111,9 @@ struct hfx_ctx {
   int debug_blocks = 0;
   int profile = 0;
   double* d_eblk = nullptr;     // integral blocks of the block class kernels
-  double* d_vmax = nullptr;     // their partial max|e|, kMaxCombo rows of kMaxChunk
+  double* d_vmax = nullptr;     // their partial max|e|, kMaxCombo rows of vmax_rows
+  long long eblk_doubles = 0;   // capacity of d_eblk
+  long long vmax_rows = 0;      // quartets per row of d_vmax
   size_t max_shm = 0;           // LDS bytes a workgroup may use
   int class_ran[2][HFX_NCLASSES] = {};
   hipEvent_t cev[2][HFX_NCLASSES][2] = {};
@@ -244,7 +246,7 @@ int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   return HFX_OK;
 }
 
-const long long kEblkDoubles = (1ll << 18) * 625;  // integral buffer of the block classes (1.3 GB)
+const long long kEblkDoubles = (1ll << 18) * 625;  // integral buffer of the block classes, at most (1.3 GB)
 const long long kMaxChunk = 1 << 20;                 // quartets per chunk, at most
 const int kMaxCombo = 16;                            // combination kernels per class, at most
 
@@ -257,12 +259,23 @@ const int kMaxCombo = 16;                            // combination kernels per 
 int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long long n, double* dump) {
   const ClassKernels& k = kClassKernels[cls];
   if (n <= 0) return HFX_OK;
-  if (!c->d_eblk) {
-    if (dalloc(&c->d_eblk, (size_t)kEblkDoubles)) return HFX_E_NOMEM;
-    if (dalloc(&c->d_vmax, (size_t)kMaxChunk * kMaxCombo)) return HFX_E_NOMEM;
-  }
   const long long ne = (long long)k.dims[0] * k.dims[1] * k.dims[2] * k.dims[3];
-  const long long chunk = std::min(kMaxChunk, (kEblkDoubles / ne) / HFX_NT * HFX_NT);
+  // The integral buffer grows to what this class needs for all its quartets
+  // at once (small molecules take megabytes), up to kEblkDoubles; larger
+  // lists are chunked.  Growing frees the old buffer (an implicit device
+  // synchronisation), so it grows at least twofold.
+  const long long need = std::min(kEblkDoubles, ((n + HFX_NT - 1) / HFX_NT) * HFX_NT * ne);
+  if (need > c->eblk_doubles) {
+    const long long cap = std::min(kEblkDoubles, std::max(need, 2 * c->eblk_doubles));
+    const long long rows = std::min(kMaxChunk, (cap / 16 + HFX_NT - 1) / HFX_NT * HFX_NT);
+    if (c->d_eblk) { HIPOK(hipFree(c->d_eblk)); c->d_eblk = nullptr; }
+    if (c->d_vmax) { HIPOK(hipFree(c->d_vmax)); c->d_vmax = nullptr; }
+    c->eblk_doubles = c->vmax_rows = 0;
+    if (dalloc(&c->d_eblk, (size_t)cap) || dalloc(&c->d_vmax, (size_t)rows * kMaxCombo)) return HFX_E_NOMEM;
+    c->eblk_doubles = cap;
+    c->vmax_rows = rows;
+  }
+  const long long chunk = std::min(c->vmax_rows, (c->eblk_doubles / ne) / HFX_NT * HFX_NT);
   const int K = prec == 1 ? 5 : 8;
   // LDS rows of X for the digestion (off for the diagonal builds: every bra once)
   const size_t rows_shm = (size_t)(k.dims[0] + k.dims[1]) * c->nb * sizeof(double);

@@ -217,9 +217,12 @@ class Gen:
         W("    const int aoA = pb.ao%s, aoB = pb.ao%s, aoC = pk.ao%s, aoD = pk.ao%s;" % (oa, ob, oc, od))
         W("    (void)shA; (void)shB; (void)shC; (void)shD; (void)aoA; (void)aoB; (void)aoC; (void)aoD;")
 
-    def emit_group(self, W, combos, ops_prim, ops_bra, ops_ctr, gid):
+    def emit_group(self, W, combos, ops_prim, ops_bra, ops_ctr, gid, sink=None):
         """Primitive loops + contraction + HRR + spherical transform for the
-        part combinations `combos`; returns {(ia,ib,ic,id): element var}."""
+        part combinations `combos`; returns {(ia,ib,ic,id): element var}.
+        With `sink` (block kernels) the HRR runs ket first, then the bra per
+        ket component, and sink(ia, ib, ic, id, var) is called as soon as an
+        element exists, so that it can be stored and dropped at once."""
         wbra, wket = self.weights()
         bps = sorted({c[0] for c in combos})
         kps_by_bp = {bp: [c[1] for c in combos if c[0] == bp] for bp in bps}
@@ -304,8 +307,10 @@ class Gen:
         vops = Ops()
         vmap = self.gen_vrr(vlines, vops, targets)
         ops_prim.n += vops.n
-        for ln in vlines:
-            W("        " + ln)
+        # accumulation statements per VRR variable, emitted right after the
+        # variable's definition so that a target needs no register beyond its
+        # last use in the recursion
+        accum = {}
         for kp in kps:
             wk = wket(kp)
             if two_level:
@@ -315,9 +320,10 @@ class Gen:
                             for f in cart(lf):
                                 v = vmap[(e, f)]
                                 if wk:
-                                    W("        %s += %s * %s;" % (tmp_name(kp, e, f), wk, v)); ops_prim.n += 2
+                                    st = "%s += %s * %s;" % (tmp_name(kp, e, f), wk, v); ops_prim.n += 2
                                 else:
-                                    W("        %s += %s;" % (tmp_name(kp, e, f), v)); ops_prim.n += 1
+                                    st = "%s += %s;" % (tmp_name(kp, e, f), v); ops_prim.n += 1
+                                accum.setdefault(v, []).append(st)
             else:
                 bp = bps[0]
                 if kp not in kps_by_bp[bp]:
@@ -329,9 +335,20 @@ class Gen:
                             for f in cart(lf):
                                 v = vmap[(e, f)]
                                 if wk:
-                                    W("        %s += %s * %s;" % (acc_name(bp, kp, e, f), wk, v)); ops_prim.n += 2
+                                    st = "%s += %s * %s;" % (acc_name(bp, kp, e, f), wk, v); ops_prim.n += 2
                                 else:
-                                    W("        %s += %s;" % (acc_name(bp, kp, e, f), v)); ops_prim.n += 1
+                                    st = "%s += %s;" % (acc_name(bp, kp, e, f), v); ops_prim.n += 1
+                                accum.setdefault(v, []).append(st)
+        defined = {ln.split()[2] for ln in vlines}
+        for v in list(accum):
+            if v not in defined:           # a Boys term s<m> itself
+                for st in accum.pop(v):
+                    W("        " + st)
+        for ln in vlines:
+            W("        " + ln)
+            for st in accum.pop(ln.split()[2], []):
+                W("        " + st)
+        assert not accum
         W("      }")
         if two_level:
             for (bp, kp) in combos:
@@ -349,6 +366,10 @@ class Gen:
         # ---- HRR + spherical per part combination ----
         elem = {}
         cA, cB, cC, cD = comps(self.A), comps(self.B), comps(self.C), comps(self.D)
+        if sink is not None:
+            for (bp, kp) in combos:
+                self.emit_hrr_ket_first(W, bp, kp, acc_name, ops_ctr, gid, sink, elem)
+            return elem
         for (bp, kp) in combos:
             la, lb, _ = self.erange(bp)
             lc, ld, fr = self.frange(kp)
@@ -435,6 +456,90 @@ class Gen:
                                 ops_ctr.n += 2 * len(terms) - 1
                             elem[(ia, ib, ic, id_)] = nm
         return elem
+
+    def emit_hrr_ket_first(self, W, bp, kp, acc_name, ops_ctr, gid, sink, elem):
+        """HRR on the ket for every bra index e (with the ket's spherical
+        transform), then per ket component the HRR on the bra, its spherical
+        transform, and the element handed to `sink`:
+          (e0|c,d+1_i) = (e0|c+1_i,d) + CD_i (e0|c,d),  base (e0|f0) = contracted VRR;
+          (a,b+1_i|cd) = (a+1_i,b|cd) + AB_i (a,b|cd),  base (e0|cd)."""
+        cA, cB, cC, cD = comps(self.A), comps(self.B), comps(self.C), comps(self.D)
+        la, lb, er = self.erange(bp)
+        tag = "%s%d%d%d%d" % (gid, bp[0], bp[1], kp[0], kp[1])
+        el = [e for le in er for e in cart(le)]
+        kets = [(ic, cc, id_, cd) for ic, cc in enumerate(cC) if cc[0] == kp[0]
+                for id_, cd in enumerate(cD) if cd[0] == kp[1]]
+        ketname = {}
+        for e_ in el:
+            km = {}
+
+            def hk(c_, d_, e_=e_, km=km):
+                key = (c_, d_)
+                if key in km:
+                    return km[key]
+                if sum(d_) == 0:
+                    km[key] = acc_name(bp, kp, e_, c_)
+                    return km[key]
+                i = next(k for k in range(3) if d_[k] > 0)
+                x1 = hk(add1(c_, i), sub1(d_, i))
+                x2 = hk(c_, sub1(d_, i))
+                nm = "hk%s_%s_%s_%s" % (tag, tname(e_), tname(c_), tname(d_))
+                W("    const R %s = %s + CD%s * %s;" % (nm, x1, AX[i], x2))
+                ops_ctr.n += 2
+                km[key] = nm
+                return nm
+
+            for (ic, cc, id_, cd) in kets:
+                tc_ = [(cc[2], "1")] if cc[1] == "c" else SPH[cc[2]]
+                td_ = [(cd[2], "1")] if cd[1] == "c" else SPH[cd[2]]
+                terms = [(hk(x, y), fx, fy) for (x, fx) in tc_ for (y, fy) in td_]
+                if len(terms) == 1 and terms[0][1] == "1" and terms[0][2] == "1":
+                    ketname[(e_, ic, id_)] = terms[0][0]
+                else:
+                    nm = "ks%s_%s_%d_%d" % (tag, tname(e_), ic, id_)
+                    ex = " + ".join("(R)(%s*%s) * %s" % (fx, fy, t) if (fx, fy) != ("1", "1") else t
+                                    for (t, fx, fy) in terms)
+                    W("    const R %s = %s;" % (nm, ex))
+                    ops_ctr.n += 2 * len(terms) - 1
+                    ketname[(e_, ic, id_)] = nm
+        for (ic, cc, id_, cd) in kets:
+            hm = {}
+
+            def hb(a_, b_, ic=ic, id_=id_, hm=hm):
+                key = (a_, b_)
+                if key in hm:
+                    return hm[key]
+                if sum(b_) == 0:
+                    hm[key] = ketname[(a_, ic, id_)]
+                    return hm[key]
+                i = next(k for k in range(3) if b_[k] > 0)
+                x1 = hb(add1(a_, i), sub1(b_, i))
+                x2 = hb(a_, sub1(b_, i))
+                nm = "hb%s_%d_%d_%s_%s" % (tag, ic, id_, tname(a_), tname(b_))
+                W("    const R %s = %s + AB%s * %s;" % (nm, x1, AX[i], x2))
+                ops_ctr.n += 2
+                hm[key] = nm
+                return nm
+
+            for ia, ca in enumerate(cA):
+                if ca[0] != bp[0]:
+                    continue
+                for ib, cb in enumerate(cB):
+                    if cb[0] != bp[1]:
+                        continue
+                    ta = [(ca[2], "1")] if ca[1] == "c" else SPH[ca[2]]
+                    tb = [(cb[2], "1")] if cb[1] == "c" else SPH[cb[2]]
+                    terms = [(hb(x, y), fx, fy) for (x, fx) in ta for (y, fy) in tb]
+                    nm = "e_%d_%d_%d_%d" % (ia, ib, ic, id_)
+                    if len(terms) == 1 and terms[0][1] == "1" and terms[0][2] == "1":
+                        W("    const R %s = %s;" % (nm, terms[0][0]))
+                    else:
+                        ex = " + ".join("(R)(%s*%s) * %s" % (fx, fy, t) if (fx, fy) != ("1", "1") else t
+                                        for (t, fx, fy) in terms)
+                        W("    const R %s = %s;" % (nm, ex))
+                        ops_ctr.n += 2 * len(terms) - 1
+                    elem[(ia, ib, ic, id_)] = nm
+                    sink(ia, ib, ic, id_, nm)
 
     def ref_index(self, ia, ib, ic, id_):
         n_ref = [NCOMP[x] for x in self.cls]
@@ -562,14 +667,16 @@ class Gen:
             W("    const int2 bk = a.list[iq];")
             W("    double* out = a.eblk + (size_t)((iq - a.q0) >> 6) * %d + ((iq - a.q0) & 63);" % (nA * nB * nC * nD * 64))
             self.emit_quartet_setup(W)
-            el = self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%d" % gi)
             # stored value: the diagonal builds drop |e| <= 1e-10 (fock_build_diag.f90:22);
             # the partial max|e| of the raw values feeds the block test
             W("    const bool ecut = a.diag != 0;")
             W("    double vm = 0;")
-            for (ia, ib, ic, id_), nm in sorted(el.items()):
+
+            def store(ia, ib, ic, id_, nm):
                 W("    { double o = (double)%s; vm = fmax(vm, fabs(o)); if (ecut && fabs(o) <= HFX_INTTHRESH) o = 0;"
                   " out[%d] = o; }" % (nm, (((ia * nB + ib) * nC + ic) * nD + id_) * 64))
+
+            el = self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%d" % gi, sink=store)
             W("    a.vmax[(size_t)%d * a.vstride + (iq - a.q0)] = vm;" % gi)
             n += len(el)
             W("  }")

@@ -247,3 +247,112 @@ def c60(r66=1.40, r56=1.45):
     atoms = np.array(atoms)
     assert atoms.shape == (60, 3)
     return Molecule(["c"] * 60, atoms, "c60")
+
+
+# bond lengths (Angstrom) and the sp3 angle used by the chain generator
+BOND = {"cc": 1.54, "ch": 1.09, "co": 1.43, "oh": 0.96, "cn": 1.47, "nh": 1.01}
+TETRA = math.acos(-1.0 / 3.0)     # 109.47 degrees
+VALENCE = {"c": 4, "n": 3, "o": 2}
+
+
+def _bond(a, b):
+    return BOND["".join(sorted((a, b), key="cnoh".index))]
+
+
+def _rotate(w, u, ang):
+    """w rotated by `ang` about the unit axis u (Rodrigues)."""
+    return w * math.cos(ang) + np.cross(u, w) * math.sin(ang) + u * np.dot(u, w) * (1.0 - math.cos(ang))
+
+
+def chain(nheavy, kind="alkane", jitter=0.0, rng=None):
+    """All-trans chain molecule with ideal sp3 geometry, in Angstrom.
+
+    kind: "alkane" (C_n H_2n+2, n = nheavy), "alkanol" (C_n-1 H_2n-1 OH) or
+    "amine" (C_n-1 H_2n-1 NH2); the heavy atoms form a planar zigzag in the
+    xy-plane, hydrogens complete the tetrahedra.  `jitter` adds seeded
+    Gaussian noise (sigma in Angstrom) to every coordinate.
+    """
+    if nheavy < 1:
+        raise ValueError("nheavy >= 1")
+    heavy = ["c"] * nheavy
+    if kind == "alkanol":
+        heavy[-1] = "o"
+    elif kind == "amine":
+        heavy[-1] = "n"
+    elif kind != "alkane":
+        raise ValueError("kind must be alkane, alkanol or amine")
+    if nheavy == 1 and kind != "alkane":
+        raise ValueError("an alkanol or amine needs a carbon")
+    half = TETRA / 2.0
+
+    def step(k):   # zigzag bond direction from heavy atom k-1 to k (also for the virtual ends)
+        s = 1.0 if k % 2 else -1.0
+        return np.array([math.sin(half), s * math.cos(half), 0.0])
+
+    pos = [np.zeros(3)]
+    for k in range(1, nheavy):
+        pos.append(pos[-1] + _bond(heavy[k - 1], heavy[k]) * step(k))
+    labels, xyz = list(heavy), [p.copy() for p in pos]
+    for k, el in enumerate(heavy):
+        nbrs = []
+        if k > 0:
+            nbrs.append(-step(k))
+        if k + 1 < nheavy:
+            nbrs.append(step(k + 1))
+        nh = VALENCE[el] - len(nbrs)
+        if len(nbrs) == 2:
+            b = -(nbrs[0] + nbrs[1]); b /= np.linalg.norm(b)
+            n = np.cross(nbrs[0], nbrs[1]); n /= np.linalg.norm(n)
+            dirs = [b * math.cos(half) + n * math.sin(half), b * math.cos(half) - n * math.sin(half)]
+        elif len(nbrs) == 1:
+            u = nbrs[0]
+            # first hydrogen where the zigzag would continue (trans), then +-120 deg about the bond
+            w0 = -step(k) if k == 0 else step(k + 1)
+            dirs = [w0, _rotate(w0, u, 2.0 * PI / 3.0), _rotate(w0, u, -2.0 * PI / 3.0)]
+            if el == "n":
+                dirs = dirs[1:]
+        else:
+            t = 1.0 / math.sqrt(3.0)
+            dirs = [np.array(v) * t for v in ((1, 1, 1), (1, -1, -1), (-1, 1, -1), (-1, -1, 1))]
+        for d in dirs[:nh]:
+            labels.append("h")
+            xyz.append(pos[k] + _bond(el, "h") * d)
+    xyz = np.array(xyz)
+    if jitter > 0.0:
+        rng = rng if rng is not None else np.random.default_rng(0)
+        xyz = xyz + rng.normal(0.0, jitter, xyz.shape)
+    name = {"alkane": "C%dH%d" % (nheavy, 2 * nheavy + 2),
+            "alkanol": "C%dH%dOH" % (nheavy - 1, 2 * nheavy - 1),
+            "amine": "C%dH%dNH2" % (nheavy - 1, 2 * nheavy - 1)}[kind]
+    return Molecule(labels, xyz, name)
+
+
+def batch_molecules(n=512, seed=2024, hmin=3, hmax=20, jitter=0.05):
+    """The batched workload: n chain molecules, kinds alkane / alkanol / amine
+    in turn, heavy-atom counts uniform in [hmin, hmax], jittered (sigma 0.05 A)."""
+    rng = np.random.default_rng(seed)
+    kinds = ("alkane", "alkanol", "amine")
+    out = []
+    for i in range(n):
+        nh = int(rng.integers(hmin, hmax + 1))
+        out.append(chain(nh, kinds[i % 3], jitter=jitter, rng=rng))
+    return out
+
+
+def serving_requests(rate=20.0, duration=300.0, seed=7, amin=5, amax=120):
+    """The serving workload: Poisson arrivals (`rate` per second over
+    `duration` s); each request a chain molecule whose atom count is
+    log-uniform in [amin, amax], basis STO-3G or 6-31G(d) with equal odds.
+    Returns [(arrival_s, Molecule, basis_name)]."""
+    rng = np.random.default_rng(seed)
+    kinds = ("alkane", "alkanol", "amine")
+    t, out = 0.0, []
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= duration:
+            return out
+        natom = math.exp(rng.uniform(math.log(amin), math.log(amax)))
+        kind = kinds[int(rng.integers(0, 3))]
+        nheavy = max(1 if kind == "alkane" else 2, int(round((natom - 2.0) / 3.0)))
+        basis = "sto-3g" if rng.random() < 0.5 else "6-31g(d)"
+        out.append((t, chain(nheavy, kind, jitter=0.05, rng=rng), basis))

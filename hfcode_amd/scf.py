@@ -49,8 +49,11 @@ class SCFResult:
 
 
 def rhf(builder, S, H, nocc, unuc, maxiter=MAXITER, ndiis=NDIIS, conv_p=CONV_P, conv_e=CONV_E,
-        lowthresh=LOWTHRESH, intthresh=INTTHRESH, P0=None, levelshift=None, log=None):
-    """Run the direct SCF; returns an SCFResult (energy, P, F, iterations, converged)."""
+        lowthresh=LOWTHRESH, intthresh=INTTHRESH, P0=None, levelshift=None, log=None, diis_start=0.3):
+    """Run the direct SCF; returns an SCFResult (energy, P, F, iterations, converged).
+
+    `diis_start`: max|DIIS error| below which DIIS starts and the level shift
+    stops (0.3 in the reference, direct_hartreefock.f90:146-150)."""
     nb = S.shape[0]
     w, U = np.linalg.eigh(S)
     X = (U * (1.0 / np.sqrt(w))) @ U.T                     # S^-1/2
@@ -115,7 +118,7 @@ def rhf(builder, S, H, nocc, unuc, maxiter=MAXITER, ndiis=NDIIS, conv_p=CONV_P, 
         resmax = float(np.abs(resid).max())
         fold[0] = fob
         eo[0] = eelec
-        if it >= 2 and resmax < 0.3 and not diis:
+        if it >= 2 and resmax < diis_start and not diis:
             diis = True
             uselevshft = False
         if it >= 7 and thresh > intthresh and resmax < 0.005:
