@@ -180,6 +180,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["c60", "batch", "serve"], default="c60")
+    ap.add_argument("--chunk", type=int, default=0, help="quartets per block-class chunk (0: library default)")
     ap.add_argument("--batch-n", type=int, default=512)
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--rate", type=float, default=20.0)
@@ -203,6 +204,8 @@ def main():
     dH = torch.from_numpy(np.ascontiguousarray(H)).to(dev)
     dF = torch.empty_like(dH)
     fb.set_profile(True)
+    if a.chunk:
+        fb.set_chunk(a.chunk)
     # FP64
     fb.set_mixed(0.0)
     dt64, cms64, cfl64, st64 = time_builds(fb, dP, dH, dF, a.thr, a.steps, a.warmup, torch)

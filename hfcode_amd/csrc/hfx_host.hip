@@ -115,6 +115,7 @@ struct hfx_ctx {
   long long eblk_doubles = 0;   // capacity of d_eblk
   long long vmax_rows = 0;      // quartets per row of d_vmax
   size_t max_shm = 0;           // LDS bytes a workgroup may use
+  long long chunk_opt = 0;      // HFX_OPT_CHUNK (0: as large as the buffer allows)
   int class_ran[2][HFX_NCLASSES] = {};
   hipEvent_t cev[2][HFX_NCLASSES][2] = {};
   int last_kind = 0;  // 0 none, 1 non-diag, 2 diag
@@ -246,8 +247,8 @@ int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   return HFX_OK;
 }
 
-const long long kEblkDoubles = (1ll << 18) * 625;  // integral buffer of the block classes, at most (1.3 GB)
-const long long kMaxChunk = 1 << 20;                 // quartets per chunk, at most
+const long long kEblkDoubles = (1ll << 20) * 625;  // integral buffer of the block classes, at most (5.2 GB)
+const long long kMaxChunk = 1 << 21;                 // quartets per chunk, at most
 const int kMaxCombo = 16;                            // combination kernels per class, at most
 
 // Heavy classes: combination kernels (integrals to HBM, tiles of 64
@@ -275,7 +276,8 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     c->eblk_doubles = cap;
     c->vmax_rows = rows;
   }
-  const long long chunk = std::min(c->vmax_rows, (c->eblk_doubles / ne) / HFX_NT * HFX_NT);
+  long long chunk = std::min(c->vmax_rows, (c->eblk_doubles / ne) / HFX_NT * HFX_NT);
+  if (c->chunk_opt > 0) chunk = std::min(chunk, c->chunk_opt);
   const int K = prec == 1 ? 5 : 8;
   // LDS rows of X for the digestion (off for the diagonal builds: every bra once)
   const size_t rows_shm = (size_t)(k.dims[0] + k.dims[1]) * c->nb * sizeof(double);
@@ -730,6 +732,12 @@ int hfx_set_option(hfx_ctx* c, int key, double value) {
   }
   if (key == HFX_OPT_DEBUG_BLOCKS) {
     c->debug_blocks = value != 0.0;
+    return HFX_OK;
+  }
+  if (key == HFX_OPT_CHUNK) {
+    if (!(value >= 0.0) || value > 1e12) return HFX_E_ARG;
+    c->chunk_opt = (long long)value / HFX_NT * HFX_NT;
+    if (value > 0.0 && c->chunk_opt == 0) c->chunk_opt = HFX_NT;
     return HFX_OK;
   }
   if (key == HFX_OPT_PROFILE) {

@@ -105,9 +105,10 @@ __device__ inline unsigned long long wave_sum(unsigned long long a) {
 // no quartet past it can survive (the product is monotone in each factor);
 // a 64-ary search finds that position.  The survivors of the bra are then
 // marked in bit masks (one word per 64 kets), the bra's range in each list
-// is reserved with one atomic, and the survivors are written there in ket
-// order: every bra's quartets are contiguous in its list, which the
-// digestion of the block classes relies on to accumulate rows of X in LDS.
+// is reserved with one atomic, and the survivors are written there grouped
+// by ket primitive count: every bra's quartets are contiguous in its list,
+// which the digestion of the block classes relies on to accumulate rows of
+// X in LDS.
 __global__ void k_screen(const ScreenArgs s) {
   const int lane = __lane_id();
   const int nw = gridDim.x * (blockDim.x / HFX_WAVE);
@@ -187,16 +188,41 @@ __global__ void k_screen(const ScreenArgs s) {
     base32 = __shfl(base32, 0);
     wave_sync();   // the masks lane 0 stored are read by every lane
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int j0 = 0; j0 < jcut && (n64 | n32); j0 += HFX_WAVE) {
-      const unsigned long long m64 = mrow[(j0 / HFX_WAVE) * 2];
-      const unsigned long long m32 = mrow[(j0 / HFX_WAVE) * 2 + 1];
-      if ((m64 | m32) >> lane & 1ull) {
-        const int2 v = make_int2(gb, s.ket[j0 + lane]);
-        if (m64 >> lane & 1ull) s.list64[base64 + __popcll(m64 & below)] = v;
-        else s.list32[base32 + __popcll(m32 & below)] = v;
+    // Survivors grouped by ket primitive count, largest first (ket order
+    // inside a group): the class kernels loop over a quartet's primitive
+    // pairs, and a wave then runs quartets of equal trip count together.
+    int cur = 1 << 30;
+    while (n64 | n32) {
+      int nxt = 0;   // largest primitive count (+1) below cur among the survivors
+      for (int j0 = 0; j0 < jcut; j0 += HFX_WAVE) {
+        const unsigned long long m = mrow[(j0 / HFX_WAVE) * 2] | mrow[(j0 / HFX_WAVE) * 2 + 1];
+        if (m >> lane & 1ull) {
+          const int4 k = s.kinfo[j0 + lane];
+          const int key = (s.all_prims ? k.w : k.z) + 1;
+          if (key < cur) nxt = max(nxt, key);
+        }
       }
-      base64 += __popcll(m64);
-      base32 += __popcll(m32);
+      for (int o = HFX_WAVE / 2; o > 0; o >>= 1) nxt = max(nxt, __shfl_xor(nxt, o));
+      if (nxt == 0) break;
+      for (int j0 = 0; j0 < jcut; j0 += HFX_WAVE) {
+        const unsigned long long m64 = mrow[(j0 / HFX_WAVE) * 2];
+        const unsigned long long m32 = mrow[(j0 / HFX_WAVE) * 2 + 1];
+        bool sel = false;
+        if ((m64 | m32) >> lane & 1ull) {
+          const int4 k = s.kinfo[j0 + lane];
+          sel = (s.all_prims ? k.w : k.z) + 1 == nxt;
+        }
+        const unsigned long long s64 = __ballot(sel && (m64 >> lane & 1ull));
+        const unsigned long long s32 = __ballot(sel && !(m64 >> lane & 1ull));
+        if (sel) {
+          const int2 v = make_int2(gb, s.ket[j0 + lane]);
+          if (m64 >> lane & 1ull) s.list64[base64 + __popcll(s64 & below)] = v;
+          else s.list32[base32 + __popcll(s32 & below)] = v;
+        }
+        base64 += __popcll(s64);
+        base32 += __popcll(s32);
+      }
+      cur = nxt;
     }
   }
 }

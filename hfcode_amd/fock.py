@@ -28,7 +28,7 @@ CLASSES = ("ssss sssp sssd sssl sspp sspd sspl ssdd ssdl ssll spsp spsd spsl spp
 NTYPES = 10
 NCLASSES = 46
 HFX_JK, HFX_J_ONLY = 0, 1
-OPT_MIXED_SPLIT, OPT_DEBUG_BLOCKS, OPT_PROFILE = 1, 2, 3
+OPT_MIXED_SPLIT, OPT_DEBUG_BLOCKS, OPT_PROFILE, OPT_CHUNK = 1, 2, 3, 4
 ERRORS = {-1: "bad argument", -2: "unsupported basis", -3: "HIP runtime error",
           -4: "device out of memory", -5: "Schwarz tables missing (run initial_diag first)"}
 
@@ -238,6 +238,10 @@ class FockBuilder:
     def set_profile(self, on=True):
         """Time every class kernel with HIP events (stats()['class_ms'])."""
         _check(lib().hfx_set_option(self._h, OPT_PROFILE, 1.0 if on else 0.0))
+
+    def set_chunk(self, quartets):
+        """Quartets per chunk of the block classes (0: as large as possible)."""
+        _check(lib().hfx_set_option(self._h, OPT_CHUNK, float(quartets)))
 
     def set_debug_blocks(self, on=True):
         _check(lib().hfx_set_option(self._h, OPT_DEBUG_BLOCKS, 1.0 if on else 0.0))

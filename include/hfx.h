@@ -33,6 +33,10 @@ extern "C" {
                                    are evaluated in FP32 (accumulation stays FP64); 0 = all FP64 */
 #define HFX_OPT_DEBUG_BLOCKS 2  /* 1: keep every integral block of the next build (tests) */
 #define HFX_OPT_PROFILE 3       /* 1: time every class kernel with HIP events (hfx_stats.class_ms) */
+#define HFX_OPT_CHUNK 4         /* quartets per chunk of the block classes (integrals stored, then
+                                   digested), rounded down to a multiple of 256; 0 = largest the
+                                   integral buffer holds (default).  A tuning knob: results do not
+                                   depend on it. */
 
 /* pair types, in the order of the reference's pair lists (molprops.f90:122-131) */
 #define HFX_NPAIRTYPES 10       /* ss sp sd sl pp pd pl dd dl ll */
