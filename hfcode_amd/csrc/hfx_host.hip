@@ -281,7 +281,10 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   const int K = prec == 1 ? 5 : 8;
   // LDS rows of X for the digestion (off for the diagonal builds: every bra once)
   const size_t rows_shm = (size_t)(k.dims[0] + k.dims[1]) * c->nb * sizeof(double);
-  const bool rows = !a.diag && rows_shm <= c->max_shm;
+  // staging of X_cd: 16 quartets x nC nD doubles per wave
+  const size_t tb_shm = (size_t)(HFX_NT / HFX_WAVE) * std::min(HFX_WAVE, 16) * k.dims[2] * k.dims[3] * sizeof(double);
+  const bool rows = !a.diag && rows_shm + tb_shm <= c->max_shm;
+  const size_t dig_shm = (rows ? rows_shm : 0) + tb_shm;
   if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
   for (long long q0 = 0; q0 < n; q0 += chunk) {
     const long long q1 = std::min(n, q0 + chunk);
@@ -316,14 +319,14 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     g.q0 = (int)q0;
     g.q1 = (int)q1;
     g.lds_rows = rows ? 1 : 0;
+    g.tb_offset = rows ? (int)rows_shm : 0;
     for (int i = 0; i < 4; ++i) g.dims[i] = k.dims[i];
     g.swap_bra = k.swap_bra;
     g.swap_ket = k.swap_ket;
     void* gargs[] = {&g};
     // with LDS rows: two workgroups per CU, each a contiguous range of steps
     const int grid = rows ? grid_for(q1 - q0, HFX_NT, c->ncu * 2) : grid_for(q1 - q0, HFX_NT, c->ncu * 8);
-    HIPOK(hipLaunchKernel(jonly ? k.digj : k.dig, dim3(grid), dim3(HFX_NT), gargs, rows ? rows_shm : 0,
-                          c->stream));
+    HIPOK(hipLaunchKernel(jonly ? k.digj : k.dig, dim3(grid), dim3(HFX_NT), gargs, dig_shm, c->stream));
   }
   if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));
   c->class_ran[prec][cls] = 1;

@@ -813,13 +813,27 @@ class Gen:
         W("    const int aoA = pb.ao%s, aoB = pb.ao%s, aoC = pk.ao%s, aoD = pk.ao%s;" % (oa, ob, oc, od))
         W("    const double w = act ? ((shA == shB) ? 0.5 : 1.0) * ((shC == shD) ? 0.5 : 1.0) * (d.diag ? 0.5 : 1.0) : 0.0;")
         W("    const double wj = 2.0 * w, wk = -0.5 * w;")
-        # pass J (each pass re-derives the block pointer so that the element
-        # addresses are not shared, and kept live, across passes)
+        ncd = nC * nD
+
+        def jab_flush(ia, ib):
+            W("      jab *= wj;")
+            W("      if (rows) {")
+            W("        double r = inl ? jab : 0.0;")
+            W("        for (int o = HFX_WAVE / 2; o > 0; o >>= 1) r += __shfl_xor(r, o);")
+            W("        if (lane == 0 && r != 0.0) atomicAdd(&srow[%d * nbf + rB + %d], r);" % (ia, ib))
+            W("      }")
+            W("      if (act && !inl) atomicAdd(&X[(size_t)(aoA + %d) * nbf + aoB + %d], jab);" % (ia, ib))
+
+        # pass J; each pass re-derives the block pointer so that the element
+        # addresses are not shared, and kept live, across passes.  (Merging
+        # J and K1 into one pass reads the block once less but needs up to
+        # 256 registers, one wave per SIMD: slower on C60.)
         W("    {")
         W("      const double* bp = blk; HFX_PIN(bp);")
         pload("Pcd", "aoC", nC, "aoD", nD)
-        zero("Jcd", nC * nD)
+        zero("Jcd", ncd)
         for ia in range(nA):
+            W("      {")
             for ib in range(nB):
                 W("      {")
                 W("      const double pab = d.P[(size_t)(aoA + %d) * nbf + aoB + %d];" % (ia, ib))
@@ -828,19 +842,34 @@ class Gen:
                     for id_ in range(nD):
                         W("      { const double v = bp[%d]; jab = fma(v, Pcd_%d_%d, jab); Jcd_%d = fma(v, pab, Jcd_%d); }" % (
                             lin(ia, ib, ic, id_), ic, id_, ic * nD + id_, ic * nD + id_))
-                W("      jab *= wj;")
-                W("      if (rows) {")
-                W("        double r = inl ? jab : 0.0;")
-                W("        for (int o = HFX_WAVE / 2; o > 0; o >>= 1) r += __shfl_xor(r, o);")
-                W("        if (lane == 0 && r != 0.0) atomicAdd(&srow[%d * nbf + rB + %d], r);" % (ia, ib))
+                jab_flush(ia, ib)
+                pin("Jcd", ncd)
                 W("      }")
-                W("      if (act && !inl) atomicAdd(&X[(size_t)(aoA + %d) * nbf + aoB + %d], jab);" % (ia, ib))
-                pin("Jcd", nC * nD)
-                W("      }")
-        W("      if (act) {")
-        for ic in range(nC):
-            for id_ in range(nD):
-                W("        atomicAdd(&X[(size_t)(aoC + %d) * nbf + aoD + %d], wj * Jcd_%d);" % (ic, id_, ic * nD + id_))
+            W("      }")
+        # X_cd through LDS: TG quartets of the wave at a time, lanes over
+        # consecutive elements, so one atomic instruction covers the row
+        # segments of a few quartets instead of 64 unrelated addresses
+        W("      {")
+        W("        constexpr int TG = HFX_WAVE < 16 ? HFX_WAVE : 16;")
+        W("        double* tb = reinterpret_cast<double*>(hfx_smem + d.tb_offset) + (threadIdx.x / HFX_WAVE) * (TG * %d);" % ncd)
+        W("        const double sc = act ? wj : 0.0;")
+        W("        for (int r = 0; r < HFX_WAVE / TG; ++r) {")
+        W("          if (lane / TG == r) {")
+        W("            double* t = tb + (lane - r * TG) * %d;" % ncd)
+        W("            " + " ".join("t[%d] = sc * Jcd_%d;" % (k, k) for k in range(ncd)))
+        W("          }")
+        W("          wave_sync();")
+        W("          for (int f0 = 0; f0 < TG * %d; f0 += HFX_WAVE) {" % ncd)
+        W("            const int f = f0 + lane;")
+        W("            const bool ok = f < TG * %d;" % ncd)
+        W("            const int ql = ok ? f / %d : 0, e = ok ? f - ql * %d : 0;" % (ncd, ncd))
+        W("            const int src = r * TG + ql;")
+        W("            const int rc = __shfl(aoC, src) + e / %d, cd = __shfl(aoD, src) + e %% %d;" % (nD, nD))
+        W("            const double v = ok ? tb[f] : 0.0;")
+        W("            if (v != 0.0) atomicAdd(&X[(size_t)rc * nbf + cd], v);")
+        W("          }")
+        W("          wave_sync();")
+        W("        }")
         W("      }")
         W("    }")
         W("    if constexpr (!JONLY) {")
