@@ -139,6 +139,7 @@ def run_batch(a):
                    "nb_mean": round(float(np.mean(nbs)), 1), "workers": a.workers, "parallelism": "single GPU"},
         "converged": conv, "iterations_mean": round(float(np.mean(its)), 2), "iterations_max": max(its),
         "solve_s_p50": round(float(np.median([r["setup_s"] + r["scf_s"] for r in res])), 4),
+        "fock_s_fraction": round(float(sum(r["fock_s"] for r in res) / sum(r["setup_s"] + r["scf_s"] for r in res)), 3),
         "wall_s": round(wall, 3),
     }
     print(json.dumps(out), flush=True)
@@ -217,6 +218,8 @@ def main():
     Fmx = dF.cpu().numpy()
     Emx = 0.5 * float(np.sum(P * (H + Fmx))) + unuc
     fb.set_mixed(0.0)
+    # one FP64 build at the tighter cut-off the reference configuration also uses
+    dt12, _, _, st12 = time_builds(fb, dP, dH, dF, 1e-12, 1, 1, torch)
     err = abs(Emx - E64)
     b64 = a.steps / dt64
     bmx = a.steps / dtmx
@@ -266,6 +269,13 @@ def main():
             "avg_launch_ms": round(float(cms64[k, 0]), 4),
         },
         "class_ms_fp64": {fock.CLASSES[i]: round(float(cms64[i, 0]), 3) for i in range(46) if cms64[i, 0] > 0},
+        "class_ms_mixed": {fock.CLASSES[i]: [round(float(cmsmx[i, 0]), 3), round(float(cmsmx[i, 1]), 3)]
+                           for i in range(46) if cmsmx[i, 0] + cmsmx[i, 1] > 0},
+        "fp32_quartet_fraction": round(float(stmx["quartets"][:, 1].sum()) / max(1.0, float(stmx["quartets"].sum())), 4),
+        "fp32_prim_quartet_fraction": round(float(stmx["prim_quartets"][:, 1].sum())
+                                            / max(1.0, float(stmx["prim_quartets"].sum())), 4),
+        "fp64_builds_per_s_thr1e-12": round(1.0 / dt12, 4),
+        "quartets_per_build_thr1e-12": int(st12["quartets"][:, 0].sum()),
     }
     out.update({k_: v for k_, v in info.items()})
     if not a.no_cpu:

@@ -83,7 +83,8 @@ def solve(mol, basis_name, maxiter=50, thr=scf.INTTHRESH, diis_start=DIIS_START)
     finally:
         fb.close()
     return dict(name=mol.title, natom=mol.natom, nb=st.nb, basis=basis_name, energy=res.energy,
-                iterations=res.iterations, converged=bool(res.converged), setup_s=t1 - t0, scf_s=t2 - t1)
+                iterations=res.iterations, converged=bool(res.converged), setup_s=t1 - t0, scf_s=t2 - t1,
+                fock_s=res.t_fock)
 
 
 def _blas_one_thread():

@@ -54,6 +54,7 @@ struct ClassArgs {
   double* blocks;          // optional: dump of every block (625 per quartet)
   double* schw_out;        // optional: sqrt(max|(ab|ab)|) per bra pair
   double* eblk;            // block kernels: integrals of quartets [q0, q1), tiles of 64 quartets, element-major
+                           // (stored as R: doubles from the FP64 kernels, floats from the FP32 ones)
   double* vmax;            // block kernels: max|e| of each combination's part, [combination][quartet - q0]
   double thr;
   int nb;

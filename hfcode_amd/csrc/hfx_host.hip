@@ -319,14 +319,18 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     g.q0 = (int)q0;
     g.q1 = (int)q1;
     g.lds_rows = rows ? 1 : 0;
-    g.tb_offset = rows ? (int)rows_shm : 0;
+    g.tb_offset = (int)(dig_shm - tb_shm);
     for (int i = 0; i < 4; ++i) g.dims[i] = k.dims[i];
     g.swap_bra = k.swap_bra;
     g.swap_ket = k.swap_ket;
     void* gargs[] = {&g};
-    // with LDS rows: two workgroups per CU, each a contiguous range of steps
-    const int grid = rows ? grid_for(q1 - q0, HFX_NT, c->ncu * 2) : grid_for(q1 - q0, HFX_NT, c->ncu * 8);
-    HIPOK(hipLaunchKernel(jonly ? k.digj : k.dig, dim3(grid), dim3(HFX_NT), gargs, dig_shm, c->stream));
+    // with LDS rows: as many workgroups as the LDS holds per CU (at most
+    // two), each a contiguous range of steps
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, c->max_shm / std::max<size_t>(dig_shm, 1)));
+    const int grid = rows ? grid_for(q1 - q0, HFX_NT, c->ncu * per_cu) : grid_for(q1 - q0, HFX_NT, c->ncu * 8);
+    // FP32 quartets: their blocks are stored as floats (digestion stays FP64)
+    const void* dfn = prec == 1 ? k.digf : (jonly ? k.digj : k.dig);
+    HIPOK(hipLaunchKernel(dfn, dim3(grid), dim3(HFX_NT), gargs, dig_shm, c->stream));
   }
   if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));
   c->class_ran[prec][cls] = 1;

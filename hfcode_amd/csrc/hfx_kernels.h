@@ -42,7 +42,8 @@ struct DiagScreenArgs {
 struct DigestArgs {
   const int2* list;            // quartet list of the class
   const int* count;            // its length
-  const double* eblk;          // integrals of quartets [q0, q1): tiles of 64 quartets, element-major, oriented layout
+  const double* eblk;          // integrals of quartets [q0, q1): tiles of 64 quartets, element-major, oriented
+                               // layout; floats for the FP32 list (read through the kernel's element type)
   const double* vmax;          // partial max|e| per combination kernel, [combination][quartet - q0]
   int vstride;
   const PairRec* pairs;

@@ -141,6 +141,8 @@ __global__ void k_screen(const ScreenArgs s) {
     unsigned long long* mrow = s.masks + (size_t)i * s.nchunk * 2;
     int n64 = 0, n32 = 0;
     unsigned long long pq64 = 0, bp64 = 0, pq32 = 0, bp32 = 0;
+    unsigned long long klo = 0, khi = 0;   // set of survivors' primitive counts + 1 (<= 101)
+    #pragma unroll 4
     for (int j0 = 0; j0 < jcut; j0 += HFX_WAVE) {
       const int j = j0 + lane;
       bool k64 = false, k32 = false;
@@ -158,6 +160,8 @@ __global__ void k_screen(const ScreenArgs s) {
           const unsigned long long nb_ = s.all_prims ? b.w : b.z;
           const unsigned long long nk_ = s.all_prims ? k.w : k.z;
           if (k64) { pq64 += nb_ * nk_; bp64 += nb_; } else { pq32 += nb_ * nk_; bp32 += nb_; }
+          const int key = (int)nk_ + 1;
+          if (key < 64) klo |= 1ull << key; else khi |= 1ull << (key - 64);
         }
       }
       const unsigned long long m64 = __ballot(k64), m32 = __ballot(k32);
@@ -169,6 +173,10 @@ __global__ void k_screen(const ScreenArgs s) {
       n32 += __popcll(m32);
     }
     pq64 = wave_sum(pq64); bp64 = wave_sum(bp64); pq32 = wave_sum(pq32); bp32 = wave_sum(bp32);
+    for (int o = HFX_WAVE / 2; o > 0; o >>= 1) {
+      klo |= __shfl_xor(klo, o);
+      khi |= __shfl_xor(khi, o);
+    }
     int base64 = 0, base32 = 0;
     if (lane == 0) {
       if (n64) {
@@ -191,26 +199,19 @@ __global__ void k_screen(const ScreenArgs s) {
     // Survivors grouped by ket primitive count, largest first (ket order
     // inside a group): the class kernels loop over a quartet's primitive
     // pairs, and a wave then runs quartets of equal trip count together.
-    int cur = 1 << 30;
-    while (n64 | n32) {
-      int nxt = 0;   // largest primitive count (+1) below cur among the survivors
-      for (int j0 = 0; j0 < jcut; j0 += HFX_WAVE) {
-        const unsigned long long m = mrow[(j0 / HFX_WAVE) * 2] | mrow[(j0 / HFX_WAVE) * 2 + 1];
-        if (m >> lane & 1ull) {
-          const int4 k = s.kinfo[j0 + lane];
-          const int key = (s.all_prims ? k.w : k.z) + 1;
-          if (key < cur) nxt = max(nxt, key);
-        }
-      }
-      for (int o = HFX_WAVE / 2; o > 0; o >>= 1) nxt = max(nxt, __shfl_xor(nxt, o));
-      if (nxt == 0) break;
+    // One scan of the bra's masks per distinct count (the set came from the
+    // mask pass).
+    while (klo | khi) {
+      const int key = khi ? 127 - __clzll((long long)khi) : 63 - __clzll((long long)klo);
+      if (key >= 64) khi &= ~(1ull << (key - 64)); else klo &= ~(1ull << key);
+      #pragma unroll 4
       for (int j0 = 0; j0 < jcut; j0 += HFX_WAVE) {
         const unsigned long long m64 = mrow[(j0 / HFX_WAVE) * 2];
         const unsigned long long m32 = mrow[(j0 / HFX_WAVE) * 2 + 1];
         bool sel = false;
         if ((m64 | m32) >> lane & 1ull) {
           const int4 k = s.kinfo[j0 + lane];
-          sel = (s.all_prims ? k.w : k.z) + 1 == nxt;
+          sel = (s.all_prims ? k.w : k.z) + 1 == key;
         }
         const unsigned long long s64 = __ballot(sel && (m64 >> lane & 1ull));
         const unsigned long long s32 = __ballot(sel && !(m64 >> lane & 1ull));
@@ -222,7 +223,6 @@ __global__ void k_screen(const ScreenArgs s) {
         base64 += __popcll(s64);
         base32 += __popcll(s32);
       }
-      cur = nxt;
     }
   }
 }

@@ -665,7 +665,7 @@ class Gen:
             W("  const int qend = min(a.q1, *a.count);")
             W("  for (int iq = a.q0 + blockIdx.x * blockDim.x + threadIdx.x; iq < qend; iq += gridDim.x * blockDim.x) {")
             W("    const int2 bk = a.list[iq];")
-            W("    double* out = a.eblk + (size_t)((iq - a.q0) >> 6) * %d + ((iq - a.q0) & 63);" % (nA * nB * nC * nD * 64))
+            W("    R* out = reinterpret_cast<R*>(a.eblk) + (size_t)((iq - a.q0) >> 6) * %d + ((iq - a.q0) & 63);" % (nA * nB * nC * nD * 64))
             self.emit_quartet_setup(W)
             # stored value: the diagonal builds drop |e| <= 1e-10 (fock_build_diag.f90:22);
             # the partial max|e| of the raw values feeds the block test
@@ -674,7 +674,7 @@ class Gen:
 
             def store(ia, ib, ic, id_, nm):
                 W("    { double o = (double)%s; vm = fmax(vm, fabs(o)); if (ecut && fabs(o) <= HFX_INTTHRESH) o = 0;"
-                  " out[%d] = o; }" % (nm, (((ia * nB + ib) * nC + ic) * nD + id_) * 64))
+                  " out[%d] = (R)o; }" % (nm, (((ia * nB + ib) * nC + ic) * nD + id_) * 64))
 
             el = self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%d" % gi, sink=store)
             W("    a.vmax[(size_t)%d * a.vstride + (iq - a.q0)] = vm;" % gi)
@@ -737,10 +737,10 @@ class Gen:
                 W("        atomicAdd(&X[(size_t)(%s + %d) * nbf + %s + %d], %s * %s_%d);" % (ao1, i, ao2, j, wname, name, j))
             W("      }")
 
-        W("template <bool JONLY>")
+        W("template <bool JONLY, typename T>")
         W("__global__ void __launch_bounds__(HFX_NT) dig_%s(const DigestArgs d) {" % cls)
         W("  extern __shared__ char hfx_smem[];")
-        W("  double* srow = reinterpret_cast<double*>(hfx_smem);   // [%d][nb]: rows A then rows B" % (nA + nB))
+        W("  double* srow = reinterpret_cast<double*>(hfx_smem);   // [%d][nb]: rows A then rows B of X" % (nA + nB))
         W("  const int qend = min(d.q1, *d.count);")
         W("  if (qend <= d.q0) return;")
         W("  const int nbf = d.nb;")
@@ -791,7 +791,7 @@ class Gen:
             W("    vmax = fmax(vmax, d.vmax[(size_t)%d * d.vstride + slot]);" % c)
         W("    if (d.schw_out != nullptr && valid) d.schw_out[bk.x] = sqrt(vmax);")
         W("    const bool act = valid && (d.diag || vmax >= d.thr);")
-        W("    const double* blk = d.eblk + (size_t)(slot >> 6) * %d + (slot & 63);" % (ne * 64))
+        W("    const T* blk = reinterpret_cast<const T*>(d.eblk) + (size_t)(slot >> 6) * %d + (slot & 63);" % (ne * 64))
         W("    if (d.dump != nullptr && valid) {")
         W("      #pragma unroll 1")
         W("      for (int e = 0; e < %d; ++e) {" % ne)
@@ -801,7 +801,7 @@ class Gen:
         i3, i4 = ("id", "ic") if self.swap_ket else ("ic", "id")
         m1, m2 = (nB, nA) if self.swap_bra else (nA, nB)
         m3 = nD if self.swap_ket else nC
-        W("        d.dump[(size_t)iq * 625 + %s + %d * (%s + %d * (%s + %d * %s))] = blk[(size_t)e * 64];" % (
+        W("        d.dump[(size_t)iq * 625 + %s + %d * (%s + %d * (%s + %d * %s))] = (double)blk[(size_t)e * 64];" % (
             i1, m1, i2, m2, i3, m3, i4))
         W("      }")
         W("    }")
@@ -814,6 +814,7 @@ class Gen:
         W("    const double w = act ? ((shA == shB) ? 0.5 : 1.0) * ((shC == shD) ? 0.5 : 1.0) * (d.diag ? 0.5 : 1.0) : 0.0;")
         W("    const double wj = 2.0 * w, wk = -0.5 * w;")
         ncd = nC * nD
+        gs = 2 if ncd <= DIG_SLICE_PAIR_MAX else 1   # slices per fence (larger ones would need 2x registers)
 
         def jab_flush(ia, ib):
             W("      jab *= wj;")
@@ -824,14 +825,16 @@ class Gen:
             W("      }")
             W("      if (act && !inl) atomicAdd(&X[(size_t)(aoA + %d) * nbf + aoB + %d], jab);" % (ia, ib))
 
-        # pass J; each pass re-derives the block pointer so that the element
-        # addresses are not shared, and kept live, across passes.  (Merging
-        # J and K1 into one pass reads the block once less but needs up to
-        # 256 registers, one wave per SIMD: slower on C60.)
+        # Pass J, then K1 and K2 below; each pass re-derives the block pointer
+        # so that element addresses are not shared, and kept live, across
+        # passes.  (Measured on C60 and dropped: J and K1 in one pass, and all
+        # of J and K in one pass with per-(a, b) partial K sums in the LDS
+        # rows -- fewer block reads, but 256 registers and one wave per SIMD,
+        # slower overall.)
         W("    {")
-        W("      const double* bp = blk; HFX_PIN(bp);")
         pload("Pcd", "aoC", nC, "aoD", nD)
         zero("Jcd", ncd)
+        W("      const T* bp = blk; HFX_PIN(bp);")
         for ia in range(nA):
             W("      {")
             for ib in range(nB):
@@ -840,10 +843,11 @@ class Gen:
                 W("      double jab = 0;")
                 for ic in range(nC):
                     for id_ in range(nD):
-                        W("      { const double v = bp[%d]; jab = fma(v, Pcd_%d_%d, jab); Jcd_%d = fma(v, pab, Jcd_%d); }" % (
+                        W("      { const double v = (double)bp[%d]; jab = fma(v, Pcd_%d_%d, jab); Jcd_%d = fma(v, pab, Jcd_%d); }" % (
                             lin(ia, ib, ic, id_), ic, id_, ic * nD + id_, ic * nD + id_))
                 jab_flush(ia, ib)
-                pin("Jcd", ncd)
+                if (ia * nB + ib + 1) % gs == 0 or (ia == nA - 1 and ib == nB - 1):
+                    pin("Jcd", ncd)
                 W("      }")
             W("      }")
         # X_cd through LDS: TG quartets of the wave at a time, lanes over
@@ -875,7 +879,7 @@ class Gen:
         W("    if constexpr (!JONLY) {")
         # pass K1
         W("    {")
-        W("      const double* bp = blk; HFX_PIN(bp);")
+        W("      const T* bp = blk; HFX_PIN(bp);")
         pload("Pbd", "aoB", nB, "aoD", nD)
         pload("Pbc", "aoB", nB, "aoC", nC)
         for ia in range(nA):
@@ -885,16 +889,17 @@ class Gen:
             for ib in range(nB):
                 for ic in range(nC):
                     for id_ in range(nD):
-                        W("      { const double v = bp[%d]; Kac_%d = fma(v, Pbd_%d_%d, Kac_%d); Kad_%d = fma(v, Pbc_%d_%d, Kad_%d); }" % (
+                        W("      { const double v = (double)bp[%d]; Kac_%d = fma(v, Pbd_%d_%d, Kac_%d); Kad_%d = fma(v, Pbc_%d_%d, Kad_%d); }" % (
                             lin(ia, ib, ic, id_), ic, ib, id_, ic, id_, ib, ic, id_))
-                pin("Kac", nC); pin("Kad", nD)
+                if (ib + 1) % gs == 0 or ib == nB - 1:
+                    pin("Kac", nC); pin("Kad", nD)
             flush_row("Kac", nC, "aoA", ia, ia, "aoC", "wk")
             flush_row("Kad", nD, "aoA", ia, ia, "aoD", "wk")
             W("      }")
         W("    }")
         # pass K2
         W("    {")
-        W("      const double* bp = blk; HFX_PIN(bp);")
+        W("      const T* bp = blk; HFX_PIN(bp);")
         pload("Pad", "aoA", nA, "aoD", nD)
         pload("Pac", "aoA", nA, "aoC", nC)
         for ib in range(nB):
@@ -904,9 +909,10 @@ class Gen:
             for ia in range(nA):
                 for ic in range(nC):
                     for id_ in range(nD):
-                        W("      { const double v = bp[%d]; Kbc_%d = fma(v, Pad_%d_%d, Kbc_%d); Kbd_%d = fma(v, Pac_%d_%d, Kbd_%d); }" % (
+                        W("      { const double v = (double)bp[%d]; Kbc_%d = fma(v, Pad_%d_%d, Kbc_%d); Kbd_%d = fma(v, Pac_%d_%d, Kbd_%d); }" % (
                             lin(ia, ib, ic, id_), ic, ia, id_, ic, id_, ia, ic, id_))
-                pin("Kbc", nC); pin("Kbd", nD)
+                if (ia + 1) % gs == 0 or ia == nA - 1:
+                    pin("Kbc", nC); pin("Kbd", nD)
             flush_row("Kbc", nC, "aoB", ib, nA + ib, "aoC", "wk")
             flush_row("Kbd", nD, "aoB", ib, nA + ib, "aoD", "wk")
             W("      }")
@@ -915,8 +921,9 @@ class Gen:
         W("  }")
         W("  if (rows && curb >= 0) flush();")
         W("}")
-        W("template __global__ void dig_%s<false>(const DigestArgs);" % cls)
-        W("template __global__ void dig_%s<true>(const DigestArgs);" % cls)
+        W("template __global__ void dig_%s<false, double>(const DigestArgs);" % cls)
+        W("template __global__ void dig_%s<true, double>(const DigestArgs);" % cls)
+        W("template __global__ void dig_%s<false, float>(const DigestArgs);" % cls)
 
     def kernel(self, split):
         L = []
@@ -977,6 +984,7 @@ class Gen:
 
 
 SPLIT_COST = 200    # classes whose fused kernel would keep more doubles live are split
+DIG_SLICE_PAIR_MAX = 16   # digestion: two block slices (nC x nD elements) per compiler fence when nC nD <= this
 
 
 def boys_flops(M, K):
@@ -1022,7 +1030,7 @@ def main(outdir):
         if table[c]["split"]:
             for k in range(table[c]["combos"]):
                 D.append("template <typename R> __global__ void eri_%s_c%d(const ClassArgs);" % (c, k))
-            D.append("template <bool JONLY> __global__ void dig_%s(const DigestArgs);" % c)
+            D.append("template <bool JONLY, typename T> __global__ void dig_%s(const DigestArgs);" % c)
         else:
             D.append("template <typename R, bool JONLY> __global__ void eri_%s(const ClassArgs);" % c)
     D.append("struct ClassKernels {")
@@ -1030,7 +1038,7 @@ def main(outdir):
     D.append("  double flops_prim, flops_prim_fp32, flops_braprim, flops_quartet;")
     D.append("  int split; int dims[4]; int swap_bra, swap_ket;")
     D.append("  int ncombo; int combo_M[16]; const void* c64[16]; const void* c32[16];")
-    D.append("  const void* dig; const void* digj;")
+    D.append("  const void* dig; const void* digj; const void* digf;")
     D.append("};")
     D.append("static const ClassKernels kClassKernels[%d] = {" % len(names))
     for c in names:
@@ -1041,11 +1049,12 @@ def main(outdir):
             cm = "{" + ", ".join(str(x) for x in t["combo_M"]) + "}"
             c64 = "{" + ", ".join("(const void*)&eri_%s_c%d<double>" % (c, k) for k in range(nc)) + "}"
             c32 = "{" + ", ".join("(const void*)&eri_%s_c%d<float>" % (c, k) for k in range(nc)) + "}"
-            dg = "(const void*)&dig_%s<false>, (const void*)&dig_%s<true>" % (c, c)
+            dg = ("(const void*)&dig_%s<false, double>, (const void*)&dig_%s<true, double>, "
+                  "(const void*)&dig_%s<false, float>" % (c, c, c))
         else:
             fns = "(const void*)&eri_%s<double, false>, (const void*)&eri_%s<double, true>, (const void*)&eri_%s<float, false>" % (c, c, c)
             nc, cm, c64, c32 = 0, "{0}", "{nullptr}", "{nullptr}"
-            dg = "nullptr, nullptr"
+            dg = "nullptr, nullptr, nullptr"
         D.append("  {%s, %d, %d, %d, %d, %d, %d, %d, {%d, %d, %d, %d}, %d, %d, %d, %s, %s, %s, %s}," % (
             fns, t["M"], int(t["all_prims"]), t["flops_prim"], t["flops_prim_fp32"], t["flops_braprim"],
             t["flops_quartet"], int(t["split"]), *t["dims"], int(t["swap_bra"]), int(t["swap_ket"]),

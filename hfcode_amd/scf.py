@@ -23,6 +23,8 @@ or anything with the same four methods.  Steps, as the reference takes them:
 
 Linear algebra is FP64 numpy (eigh for dsyev, solve for dsysv).
 """
+import time
+
 import numpy as np
 
 MAXITER = 25
@@ -84,8 +86,10 @@ def rhf(builder, S, H, nocc, unuc, maxiter=MAXITER, ndiis=NDIIS, conv_p=CONV_P, 
     converged = False
     history = []
     etot = None
+    t_fock = 0.0
     for it in range(1, maxiter + 1):
         eeold = eelec
+        tf0 = time.perf_counter()
         if fullfock and it == 1:
             f = builder.initial_diag(H, p, thresh)
             f = builder.non_diag(f, p, thresh)
@@ -104,6 +108,7 @@ def rhf(builder, S, H, nocc, unuc, maxiter=MAXITER, ndiis=NDIIS, conv_p=CONV_P, 
             nfull += 1
             if nfull == 15:
                 fullfock = True
+        t_fock += time.perf_counter() - tf0
         eelec = 0.5 * float(np.sum(p * (H + f)))
         deltae = eelec - eeold
         etot = eelec + unuc
@@ -163,4 +168,4 @@ def rhf(builder, S, H, nocc, unuc, maxiter=MAXITER, ndiis=NDIIS, conv_p=CONV_P, 
         pold = p
         p = 2.0 * cmat[:, :nocc] @ cmat[:, :nocc].T
     return SCFResult(energy=etot, P=p, F=faobold, eigs=eigs, iterations=it, converged=converged,
-                     history=history)
+                     history=history, t_fock=t_fock)

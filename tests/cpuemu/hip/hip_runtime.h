@@ -56,6 +56,7 @@ inline unsigned long long __ballot(bool p) { return p ? 1ull : 0ull; }
 inline int __lane_id() { return 0; }
 inline int __ffsll(long long m) { return m ? __builtin_ffsll(m) : 0; }
 inline int __popcll(unsigned long long m) { return __builtin_popcountll(m); }
+inline int __clzll(long long m) { return m ? __builtin_clzll((unsigned long long)m) : 64; }
 template <typename T> inline T __shfl(T v, int) { return v; }
 template <typename T> inline T __shfl_xor(T v, int) { return v; }
 template <typename T> inline T __shfl_up(T v, unsigned) { return v; }
