@@ -45,6 +45,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector, AMD public spec
+PEAK_HBM_TBS = 8.0          # MI355X HBM3E
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c60_631gd_hbm_traffic.json")
 PEAK_FP32_TFLOPS = 157.3
 
 
@@ -277,6 +279,17 @@ def main():
         "fp64_builds_per_s_thr1e-12": round(1.0 / dt12, 4),
         "quartets_per_build_thr1e-12": int(st12["quartets"][:, 0].sum()),
     }
+    # HBM bytes of the roofline kernel: FETCH_SIZE + WRITE_SIZE counter runs of this
+    # workload (rocprofv3 cannot run inside the timed process), when they match it
+    if os.path.exists(TRAFFIC_FILE) and a.density == "scf" and a.thr == 1e-10 and not a.chunk:
+        with open(TRAFFIC_FILE) as fh:
+            tr = json.load(fh)
+        if tr.get("kernel") == out["roofline"]["kernel"]:
+            t_s = out["roofline"]["avg_launch_ms"] * 1e-3
+            out["roofline"]["traffic"] = tr["traffic_bytes"]
+            out["roofline"]["traffic_source"] = tr["source"]
+            out["roofline"]["hbm_TBps"] = round(tr["traffic_bytes"] / t_s / 1e12, 3)
+            out["roofline"]["hbm_frac"] = round(tr["traffic_bytes"] / t_s / 1e12 / PEAK_HBM_TBS, 4)
     out.update({k_: v for k_, v in info.items()})
     if not a.no_cpu:
         bas = M.load_basis("6-31g(d)")
