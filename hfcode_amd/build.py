@@ -20,6 +20,7 @@ GEN = os.path.join(CSRC, "gen")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhfx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HOST_SOURCES = ("hfx_onee.cpp", "hfx_input.cpp")
 
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
@@ -53,8 +54,9 @@ def compile_one(src):
 
 def build(jobs=None, verbose=True):
     sys.path.insert(0, ROOT)
-    from hfcode_amd.gen import gen_eri
+    from hfcode_amd.gen import gen_eri, gen_onee
     gen_eri.main(GEN)
+    gen_onee.main(GEN)
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
     jobs = jobs or min(16, os.cpu_count() or 4)
@@ -67,13 +69,14 @@ def build(jobs=None, verbose=True):
             objs.append(obj)
             if err:
                 errs.append(err)
-    host_src = os.path.join(CSRC, "hfx_onee.cpp")
-    host_obj = os.path.join(OBJ, "hfx_onee.cpp.o")
-    r = subprocess.run(["g++", "-O2", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-c", host_src,
-                        "-o", host_obj], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    if r.returncode:
-        errs.append(r.stdout)
-    objs.append(host_obj)
+    for name in HOST_SOURCES:     # host-only C++: one-electron set-up, input readers
+        host_src = os.path.join(CSRC, name)
+        host_obj = os.path.join(OBJ, name + ".o")
+        r = subprocess.run(["g++", "-O2", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-c", host_src,
+                            "-o", host_obj], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode:
+            errs.append(r.stdout)
+        objs.append(host_obj)
     if errs:
         raise RuntimeError("hipcc failed:\n" + "\n".join(e[-4000:] for e in errs))
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + sorted(objs)

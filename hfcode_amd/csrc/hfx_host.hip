@@ -141,6 +141,11 @@ struct hfx_ctx {
   int4* d_act_info[HFX_NPAIRTYPES] = {nullptr};   // shells and primitive counts per active position
   double* d_pair_schw[HFX_NPAIRTYPES] = {nullptr};  // Schwarz per pair (list order)
   int2* d_diag_list[HFX_NPAIRTYPES] = {nullptr};
+  // one-electron set-up on the device (allocated on first use)
+  int4* d_onee_items = nullptr;
+  int onee_count[3][3] = {};                   // work items per (la, lb), consecutive in d_onee_items
+  int *d_sh_type = nullptr, *d_sh_np = nullptr, *d_sh_ao = nullptr;
+  double *d_sh_xyz = nullptr, *d_sh_z = nullptr, *d_sh_c = nullptr, *d_sh_cl = nullptr;
   int* d_counts = nullptr;                     // [2][HFX_NCLASSES]
   unsigned long long* d_stats = nullptr;       // [2][HFX_NCLASSES][3]
 };
@@ -719,6 +724,8 @@ void hfx_ctx_destroy(hfx_ctx* c) {
   }
   hipFree(c->d_pairs); hipFree(c->d_prims); hipFree(c->d_shells); hipFree(c->d_boys64); hipFree(c->d_boys32);
   hipFree(c->d_P); hipFree(c->d_F); hipFree(c->d_X); hipFree(c->d_maxdm); hipFree(c->d_schw_out);
+  hipFree(c->d_onee_items); hipFree(c->d_sh_type); hipFree(c->d_sh_np); hipFree(c->d_sh_ao);
+  hipFree(c->d_sh_xyz); hipFree(c->d_sh_z); hipFree(c->d_sh_c); hipFree(c->d_sh_cl);
   hipFree(c->d_counts); hipFree(c->d_stats); hipFree(c->d_eblk); hipFree(c->d_vmax); hipFree(c->d_pmax); hipFree(c->d_masks);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -775,6 +782,98 @@ int hfx_fock_diag(hfx_ctx* c, const double* P, double* F, int nb, double thr) {
   if ((r = copy_fock_host(c, P, F, nb))) return r;
   if ((r = build_diag(c, c->d_P, c->d_F, thr, false))) return r;
   return fetch_fock_host(c, F, nb);
+}
+
+// S, T, V on the device: work items per (la, lb) of the shell-pair parts,
+// one generated kernel pair per (la, lb); copies of the three matrices back.
+int hfx_ctx_onee(hfx_ctx* c, int natom, const double* charges, const double* coords, double* S, double* T,
+                 double* V) {
+  if (!c || !S || !T || !V || natom < 0 || (natom > 0 && (!charges || !coords))) return HFX_E_ARG;
+  const int nsh = c->nsh, nb = c->nb;
+  if (!c->d_onee_items) {
+    auto nparts = [](int t) { return t == 12 ? 2 : 1; };
+    auto part_l = [](int t, int p) { return t == 1 ? 0 : t == 2 ? 1 : t == 3 ? 2 : p; };
+    std::vector<int4> items[3][3];
+    for (int i = 0; i < nsh; ++i)
+      for (int j = 0; j <= i; ++j)
+        for (int pa = 0; pa < nparts(c->shells[i].type); ++pa)
+          for (int pb = 0; pb < nparts(c->shells[j].type); ++pb)
+            items[part_l(c->shells[i].type, pa)][part_l(c->shells[j].type, pb)].push_back(make_int4(i, j, pa, pb));
+    std::vector<int4> all;
+    for (int la = 0; la < 3; ++la)
+      for (int lb = 0; lb < 3; ++lb) {
+        c->onee_count[la][lb] = (int)items[la][lb].size();
+        all.insert(all.end(), items[la][lb].begin(), items[la][lb].end());
+      }
+    const int mp = 16;
+    std::vector<int> ty(nsh), np(nsh), ao(nsh);
+    std::vector<double> xyz(3 * nsh), z((size_t)mp * nsh), cc((size_t)mp * nsh), cl((size_t)mp * nsh);
+    for (int i = 0; i < nsh; ++i) {
+      const Shell& sh = c->shells[i];
+      ty[i] = sh.type;
+      np[i] = sh.nprim;
+      ao[i] = sh.ao;
+      for (int k = 0; k < 3; ++k) xyz[3 * i + k] = sh.x[k];
+      for (int k = 0; k < mp; ++k) {
+        z[(size_t)mp * i + k] = sh.zeta[k];
+        cc[(size_t)mp * i + k] = sh.djk[k];
+        cl[(size_t)mp * i + k] = sh.djkl[k];
+      }
+    }
+    if (dalloc(&c->d_onee_items, all.size()) || dalloc(&c->d_sh_type, nsh) || dalloc(&c->d_sh_np, nsh) ||
+        dalloc(&c->d_sh_ao, nsh) || dalloc(&c->d_sh_xyz, xyz.size()) || dalloc(&c->d_sh_z, z.size()) ||
+        dalloc(&c->d_sh_c, cc.size()) || dalloc(&c->d_sh_cl, cl.size()))
+      return HFX_E_NOMEM;
+    HIPOK(hipMemcpy(c->d_onee_items, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(c->d_sh_type, ty.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(c->d_sh_np, np.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(c->d_sh_ao, ao.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(c->d_sh_xyz, xyz.data(), xyz.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(c->d_sh_z, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(c->d_sh_c, cc.data(), cc.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(c->d_sh_cl, cl.data(), cl.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  double *d_q = nullptr, *d_xyz = nullptr, *d_S = nullptr, *d_T = nullptr, *d_V = nullptr;
+  const size_t nb2 = (size_t)nb * nb;
+  int r = HFX_OK;
+  if (dalloc(&d_q, std::max(natom, 1)) || dalloc(&d_xyz, 3 * std::max(natom, 1)) || dalloc(&d_S, nb2) ||
+      dalloc(&d_T, nb2) || dalloc(&d_V, nb2))
+    r = HFX_E_NOMEM;
+  if (r == HFX_OK && natom > 0) {
+    if (hipMemcpy(d_q, charges, natom * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_xyz, coords, 3 * natom * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      r = HFX_E_HIP;
+  }
+  if (r == HFX_OK) {
+    OneeArgs a;
+    memset(&a, 0, sizeof(a));
+    a.sh_type = c->d_sh_type; a.sh_np = c->d_sh_np; a.sh_ao = c->d_sh_ao; a.sh_xyz = c->d_sh_xyz;
+    a.sh_z = c->d_sh_z; a.sh_c = c->d_sh_c; a.sh_cl = c->d_sh_cl; a.maxprim = 16;
+    a.charges = d_q; a.coords = d_xyz; a.natom = natom; a.boys64 = c->d_boys64;
+    a.S = d_S; a.T = d_T; a.V = d_V; a.nb = nb;
+    int off = 0;
+    for (int la = 0; la < 3 && r == HFX_OK; ++la)
+      for (int lb = 0; lb < 3 && r == HFX_OK; ++lb) {
+        const int n = c->onee_count[la][lb];
+        if (n == 0) continue;
+        a.items = c->d_onee_items + off;
+        a.nitems = n;
+        off += n;
+        const int grid = grid_for(n, HFX_NT, c->ncu * 8);
+        const size_t shm = (size_t)HFX_BOYS_ROWS * (la + lb + 8) * sizeof(double);
+        hipLaunchKernelGGL(kOneeST[la][lb], dim3(grid), dim3(HFX_NT), 0, c->stream, a);
+        hipLaunchKernelGGL(kOneeV[la][lb], dim3(grid), dim3(HFX_NT), shm, c->stream, a);
+        if (hipGetLastError() != hipSuccess) r = HFX_E_HIP;
+      }
+  }
+  if (r == HFX_OK &&
+      (hipMemcpyAsync(S, d_S, nb2 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+       hipMemcpyAsync(T, d_T, nb2 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+       hipMemcpyAsync(V, d_V, nb2 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+       hipStreamSynchronize(c->stream) != hipSuccess))
+    r = HFX_E_HIP;
+  hipFree(d_q); hipFree(d_xyz); hipFree(d_S); hipFree(d_T); hipFree(d_V);
+  return r;
 }
 
 int hfx_fock_initial_diag(hfx_ctx* c, const double* P, double* F, int nb, double thr) {

@@ -59,6 +59,33 @@ struct DigestArgs {
   int swap_bra, swap_ket;      // oriented A is the pair's second shell (bra / ket)
 };
 
+// One-electron kernels (generated, gen/onee.hip): one thread per work item
+// (shell a, shell b, part a, part b; 0-based shells, a >= b), S/T or V block
+// of the item written to both triangles of the nb x nb matrices.
+struct OneeArgs {
+  const int4* items;
+  int nitems;
+  const int* sh_type;          // per shell: 1 s, 2 p, 3 d, 12 L
+  const int* sh_np;            // primitives
+  const int* sh_ao;            // first AO, 0-based
+  const double* sh_xyz;        // centre, bohr (nsh x 3)
+  const double* sh_z;          // exponents (nsh x maxprim)
+  const double* sh_c;          // normalised coefficients
+  const double* sh_cl;         // L shells: p / s coefficient ratio
+  int maxprim;
+  const double* charges;       // nuclei
+  const double* coords;        // natom x 3, bohr
+  int natom;
+  const double* boys64;        // HFX_BOYS_ROWS x HFX_BOYS_COLS
+  double* S;
+  double* T;
+  double* V;
+  int nb;
+};
+typedef void (*OneeKernel)(OneeArgs);
+extern const OneeKernel kOneeST[3][3];   // [la][lb]
+extern const OneeKernel kOneeV[3][3];
+
 __global__ void k_fill_f64(double* x, size_t n, double v);
 __global__ void k_fill_i32(int* x, int n, int v);
 __global__ void k_fill_u64(unsigned long long* x, int n);

@@ -115,6 +115,16 @@ def configure(L):
     L.hfx_debug_blocks.restype = ctypes.c_longlong
     L.hfx_pair_info.argtypes = [vp, ctypes.c_int, ip, ip]
     L.hfx_onee.argtypes = [ctypes.POINTER(MolDesc), ctypes.c_int, dp, dp, dp, dp, dp]
+    L.hfx_ctx_onee.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp, dp]
+    L.hfx_molecule_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
+    L.hfx_molecule_dims.argtypes = [vp, ip, ip, ip, ip, dp]
+    L.hfx_molecule_options.argtypes = [vp, ip, ip, dp, ip]
+    L.hfx_molecule_desc.argtypes = [vp, ctypes.POINTER(MolDesc)]
+    L.hfx_molecule_shells.argtypes = [vp, ip]
+    L.hfx_molecule_atoms.argtypes = [vp, dp, dp]
+    L.hfx_molecule_density_guess.argtypes = [vp, dp]
+    L.hfx_molecule_free.argtypes = [vp]
+    L.hfx_molecule_free.restype = None
     L.hfx_version.restype = ctypes.c_char_p
     return L
 
@@ -191,6 +201,61 @@ def build_desc(shells, pairs=None, nb=None):
     return d, keep, nb, lists
 
 
+class InputMolecule:
+    """A molecule set up by the library's native readers from the texts of a
+    reference input file and basis file (``hfx_molecule_read``: read_infile,
+    read_basfile with its normalisation, spread_basis).  Carries the shell
+    arrays :class:`FockBuilder` takes, the atoms for :meth:`FockBuilder.onee`,
+    E_nuc, nocc, the input's SCF options and the atomic-density guess."""
+
+    def __init__(self, input_text, basis_text):
+        L = lib()
+        h = ctypes.c_void_p()
+        _check(L.hfx_molecule_read(input_text.encode(), basis_text.encode(), ctypes.byref(h)))
+        self._h = h
+        natom, nb, nsh, nocc, unuc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(L.hfx_molecule_dims(h, ctypes.byref(natom), ctypes.byref(nb), ctypes.byref(nsh), ctypes.byref(nocc),
+                                   ctypes.byref(unuc)))
+        self.natom, self.nb, self.nsh, self.nocc, self.unuc = natom.value, nb.value, nsh.value, nocc.value, unuc.value
+        d = MolDesc()
+        _check(L.hfx_molecule_desc(h, ctypes.byref(d)))
+        n, m = self.nsh, d.maxprim
+        self.shtyp = np.ctypeslib.as_array(d.shtyp, (n,)).copy()
+        self.nzet = np.ctypeslib.as_array(d.nzet, (n,)).copy()
+        self.aosh = np.ctypeslib.as_array(d.aosh, (n,)).copy()
+        self.xsh = np.ctypeslib.as_array(d.xsh, (n, 3)).copy()
+        self.zeta = np.ctypeslib.as_array(d.zeta, (n, m)).copy()
+        self.djk = np.ctypeslib.as_array(d.djk, (n, m)).copy()
+        self.djkl = np.ctypeslib.as_array(d.djkl, (n, m)).copy()
+        self.atsh = np.zeros(n, np.int32)
+        _check(L.hfx_molecule_shells(h, _ip(self.atsh)))
+        self.charges = np.zeros(self.natom)
+        self.coords_bohr = np.zeros((self.natom, 3))
+        _check(L.hfx_molecule_atoms(h, _dp(self.charges), _dp(self.coords_bohr)))
+        densup, lev, prt, shift = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(L.hfx_molecule_options(h, ctypes.byref(densup), ctypes.byref(lev), ctypes.byref(shift),
+                                      ctypes.byref(prt)))
+        self.densup, self.prtlevl = bool(densup.value), prt.value
+        self.levelshift = shift.value if lev.value else None
+
+    def density_guess(self):
+        """Superposition of the basis file's atomic densities (nb x nb)."""
+        P = np.zeros((self.nb, self.nb))
+        _check(lib().hfx_molecule_density_guess(self._h, _dp(P)))
+        return P
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hfx_molecule_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class FockBuilder:
     """One molecule on the GPU.
 
@@ -222,13 +287,17 @@ class FockBuilder:
         except Exception:
             pass
 
-    def onee(self, charges, coords_bohr):
-        """(S, T, V) for point charges at coords (bohr); see hfx_onee."""
+    def onee(self, charges, coords_bohr, device=True):
+        """(S, T, V) for point charges at coords (bohr): on the context's GPU
+        (hfx_ctx_onee), or with device=False by the host code (hfx_onee)."""
         nb = self.nb
         ch = np.ascontiguousarray(charges, dtype=np.float64)
         xy = np.ascontiguousarray(coords_bohr, dtype=np.float64).reshape(-1, 3)
         S, T, V = (np.zeros((nb, nb)) for _ in range(3))
-        _check(lib().hfx_onee(ctypes.byref(self._desc), len(ch), _dp(ch), _dp(xy), _dp(S), _dp(T), _dp(V)))
+        if device:
+            _check(lib().hfx_ctx_onee(self._h, len(ch), _dp(ch), _dp(xy), _dp(S), _dp(T), _dp(V)))
+        else:
+            _check(lib().hfx_onee(ctypes.byref(self._desc), len(ch), _dp(ch), _dp(xy), _dp(S), _dp(T), _dp(V)))
         return S, T, V
 
     def set_mixed(self, split):

@@ -91,10 +91,14 @@ class Molecule:
         return "\n".join(lines) + "\n"
 
 
-def basis_to_dat(basis):
-    """Text of a basis in the reference's `.dat` format (read_basfile.f90:12-70), dmat=0."""
+def basis_to_dat(basis, densities=False):
+    """Text of a basis in the reference's `.dat` format (read_basfile.f90:12-70);
+    with `densities` and atomic density blocks in the basis, dmat = 1 and the
+    DENSITY section (read_basfile.f90:71-91: per element its label, then each
+    row of the block in records of at most 7 values, then ``****``)."""
     els = basis["elements"]
-    out = ["ELEMENTS %d 0" % len(els)]
+    dens = basis.get("density") if densities else None
+    out = ["ELEMENTS %d %d" % (len(els), 1 if dens else 0)]
     for label, shells in els.items():
         out.append("%d" % len(shells))
         out.append("%s 0" % label.capitalize())
@@ -106,6 +110,14 @@ def basis_to_dat(basis):
                     row.append(repr(sh["coefs_p"][k]))
                 out.append("  " + "  ".join(row))
         out.append("****")
+    if dens:
+        out.append("DENSITY MATRICES")
+        for label in els:
+            out.append(label.capitalize())
+            for row in dens[label]:
+                for k in range(0, len(row), 7):
+                    out.append(" ".join(repr(float(v)) for v in row[k:k + 7]))
+            out.append("****")
     return "\n".join(out) + "\n"
 
 

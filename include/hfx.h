@@ -129,6 +129,35 @@ int hfx_pair_info(hfx_ctx* ctx, int gpair, int* type, int* index);
  * Host set-up, once per molecule; replaces cmpt1e (cmpt1e.f90:1-258). */
 int hfx_onee(const hfx_mol_desc* desc, int natom, const double* charges, const double* coords, double* S,
              double* T, double* V);
+/* The same matrices computed on the context's GPU (generated Obara--Saika
+ * kernels, one thread per shell pair and L-shell part pair); host arrays in
+ * and out. */
+int hfx_ctx_onee(hfx_ctx* ctx, int natom, const double* charges, const double* coords, double* S, double* T,
+                 double* V);
+
+/* Molecules in the reference's own text formats (host set-up, no HIP calls).
+ * hfx_molecule_read parses an input file (read_infile.f90:12-73) and a basis
+ * file (read_basfile.f90:12-91, with the normalisation of :100-179), both
+ * given as text, and builds the shell table and pair lists as
+ * spread_basis.f90:53-150 and precalc_kabs.f90:38-90 do.  Errors:
+ * HFX_E_ARG malformed text or an element missing from the basis,
+ * HFX_E_UNSUPPORTED UHF, more than 10 primitives or a P + D + L basis
+ * (spread_basis.f90:174-177). */
+typedef struct hfx_molecule hfx_molecule;
+int hfx_molecule_read(const char* input_text, const char* basis_text, hfx_molecule** out);
+int hfx_molecule_dims(const hfx_molecule* m, int* natom, int* nb, int* nsh, int* nocc, double* unuc);
+/* USEDENSITYSUPGUESS, LEVELSHIFT flag and value, PRINT level of the input. */
+int hfx_molecule_options(const hfx_molecule* m, int* densup, int* levelshift, double* shift, int* prtlevl);
+/* Descriptor for hfx_ctx_create / hfx_onee; its arrays belong to m (valid until hfx_molecule_free). */
+int hfx_molecule_desc(const hfx_molecule* m, hfx_mol_desc* desc);
+/* 1-based atom of each shell (nsh). */
+int hfx_molecule_shells(const hfx_molecule* m, int* atsh);
+/* Nuclear charges (natom) and coordinates (natom x 3, bohr). */
+int hfx_molecule_atoms(const hfx_molecule* m, double* charges, double* coords);
+/* Superposition of the basis file's atomic densities (nb x nb), the guess of
+ * direct_hartreefock.f90:50-64; HFX_E_STATE when the basis file has none (dmat = 0). */
+int hfx_molecule_density_guess(const hfx_molecule* m, double* P);
+void hfx_molecule_free(hfx_molecule* m);
 
 const char* hfx_version(void);
 

@@ -34,11 +34,13 @@ def _one(src):
 
 def build(jobs=8):
     sys.path.insert(0, ROOT)
-    from hfcode_amd.gen import gen_eri
+    from hfcode_amd.gen import gen_eri, gen_onee
     gen_eri.main(GEN)
+    gen_onee.main(GEN)
     os.makedirs(OBJ, exist_ok=True)
     srcs = [os.path.join(CSRC, "hfx_kernels.hip"), os.path.join(CSRC, "hfx_host.hip"),
-            os.path.join(HERE, "emu_smem.cpp"), os.path.join(CSRC, "hfx_onee.cpp")]
+            os.path.join(HERE, "emu_smem.cpp"), os.path.join(CSRC, "hfx_onee.cpp"),
+            os.path.join(CSRC, "hfx_input.cpp")]
     srcs += sorted(os.path.join(GEN, f) for f in os.listdir(GEN) if f.endswith(".hip"))
     srcs.sort(key=lambda s: -os.path.getsize(s))
     with cf.ThreadPoolExecutor(jobs) as ex:

@@ -8,6 +8,19 @@ CASES = ["water_sto3g", "benzene_631gd", "water_def2svp", "formaldehyde_631gmix"
 NCOMP = {"s": 1, "p": 3, "d": 5, "l": 4}
 
 
+def molecule(name):
+    """The molecule of a golden case (geometries as in make_golden.py)."""
+    from hfcode_amd import molecule as M
+    if name.startswith("water"):
+        return M.water()
+    if name.startswith("benzene"):
+        return M.benzene()
+    if name.startswith("formaldehyde"):
+        return M.Molecule(["c", "o", "h", "h"], [[0, 0, 0], [0, 0, 1.205], [0.9428, 0, -0.5862],
+                                                 [-0.9428, 0, -0.5862]], "formaldehyde")
+    raise KeyError(name)
+
+
 class Golden:
     def __init__(self, name):
         self.name = name

@@ -10,7 +10,7 @@ import pytest
 
 from hfcode_amd import molecule as M
 from hfcode_amd import scf
-from tests.golden_io import CASES, NCOMP, Golden
+from tests.golden_io import CASES, NCOMP, Golden, molecule
 
 TOL = 1e-10
 
@@ -98,6 +98,23 @@ def test_standalone_path_and_scf(emu_fock, name):
             assert np.allclose(s1, s2, rtol=1e-12, atol=1e-15)
     res = scf.rhf(fb, S, T + V, mol.nocc, mol.nuclear_repulsion())
     assert res.converged and abs(res.energy - float(g["E_scf"])) < 1e-9
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_device_one_electron_matches_reference(emu_fock, name):
+    """Generated S/T/V kernels (hfx_ctx_onee) against the reference's S, T, V
+    (int1e / int1e_kin / int1e_nuc matrices of the golden case) and against
+    the host code (hfx_onee)."""
+    g = Golden(name)
+    mol = molecule(name)
+    fb = emu_fock.FockBuilder(g.shells, g.pairs, nb=g.nb)
+    q = mol.znuc.astype(np.float64)
+    S, T, V = fb.onee(q, mol.coords_bohr)
+    assert np.abs(S - g["S"]).max() < 1e-13 and np.abs(T - g["T"]).max() < 1e-13
+    assert np.abs(V - g["V"]).max() < 1e-12
+    for dev, host in zip((S, T, V), fb.onee(q, mol.coords_bohr, device=False)):
+        assert np.abs(dev - host).max() < 1e-11
+        assert np.array_equal(dev, dev.T)
 
 
 def test_error_codes(emu_fock):
