@@ -120,28 +120,68 @@ def cpu_reference(mol, bas, P, thr, threads):
     return dt, F
 
 
+def _device_used_bytes():
+    """Device-wide bytes in use (hipMemGetInfo through torch), or None."""
+    try:
+        import torch
+        free, total = torch.cuda.mem_get_info(0)
+        return total - free
+    except Exception:
+        return None
+
+
+def _energy_check(seq, mine):
+    """Max |E(job) - E(same job solved alone in FP64)| over the jobs both
+    lists hold (matched by position) and how many of them converged."""
+    d = [abs(a["energy"] - b["energy"]) for a, b in zip(seq, mine)
+         if a.get("converged") and b.get("converged")]
+    return (max(d) if d else None), len(d)
+
+
 def run_batch(a):
     from hfcode_amd import batch, molecule as M
     mols = M.batch_molecules(n=a.batch_n)
+    split = a.split if a.split > 0 else None
+    # one-at-a-time plain FP64 GPU solves of every 16th molecule: the
+    # sequential baseline and each sampled job's FP64 energy
+    sample = list(range(0, len(mols), max(1, len(mols) // a.seq_n)))[:a.seq_n]
+    log("batch: sequential FP64 solves of %d sampled molecules" % len(sample))
+    seq, seq_wall = batch.run_sequential([(mols[i], "def2-sv(p)") for i in sample])
+    log("batch: sequential %.1f s" % seq_wall)
+    peak = [0]
+
     def progress(done, total, el):
+        u = _device_used_bytes()
+        if u:
+            peak[0] = max(peak[0], u)
         if done % 16 == 0 or done == total:
             log("batch: %d/%d solved, %.1f s" % (done, total, el))
-    res, wall = batch.run_batch(mols, "def2-sv(p)", workers=a.workers, progress=progress)
+    res, wall = batch.run_batch(mols, "def2-sv(p)", workers=a.workers, progress=progress, split=split)
     nbs = [r["nb"] for r in res]
     its = [r["iterations"] for r in res]
     conv = sum(r["converged"] for r in res)
+    value = len(mols) / wall
+    seq_rate = len(sample) / seq_wall
+    dE, ndE = _energy_check(seq, [res[i] for i in sample])
     out = {
         "metric": "RHF solves/s, %d chain molecules def2-SV(P)" % len(mols),
-        "value": round(len(mols) / wall, 4), "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "value": round(value, 4), "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": round(1e3 * wall, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic: all-trans alkanes / alkanols / amines, 3-20 heavy atoms, "
-                                "ideal bonds + 0.05 A Gaussian jitter, seed 2024",
+        "dtype": "f64+f32" if split else "f64",
+        "data": "synthetic: all-trans alkanes / alkanols / amines, 3-20 heavy atoms, "
+                "ideal bonds + 0.05 A Gaussian jitter, seed 2024",
         "config": {"workload": "batched RHF solves (core guess, DIIS from iteration 2, thr 1e-10, <= 50 iterations)",
                    "basis": "def2-SV(P)", "molecules": len(mols), "nb_min": min(nbs), "nb_max": max(nbs),
-                   "nb_mean": round(float(np.mean(nbs)), 1), "workers": a.workers, "parallelism": "single GPU"},
+                   "nb_mean": round(float(np.mean(nbs)), 1), "workers": a.workers, "mixed_split": split,
+                   "parallelism": "single GPU"},
         "converged": conv, "iterations_mean": round(float(np.mean(its)), 2), "iterations_max": max(its),
         "solve_s_p50": round(float(np.median([r["setup_s"] + r["scf_s"] for r in res])), 4),
         "fock_s_fraction": round(float(sum(r["fock_s"] for r in res) / sum(r["setup_s"] + r["scf_s"] for r in res)), 3),
+        "sequential_fp64_solves_per_s": round(seq_rate, 4),
+        "sequential_sample": "every %dth molecule (%d), one at a time, FP64" % (sample[1] - sample[0] if len(sample) > 1 else 1, len(sample)),
+        "vs_sequential": round(value / seq_rate, 3),
+        "energy_error_max_Eh": dE, "energy_error_jobs": ndE,
+        "peak_device_bytes": peak[0] or None,
         "wall_s": round(wall, 3),
     }
     print(json.dumps(out), flush=True)
@@ -150,24 +190,44 @@ def run_batch(a):
 def run_serve(a):
     from hfcode_amd import batch, molecule as M
     reqs = M.serving_requests(rate=a.rate, duration=a.duration)
-    srv = batch.Server(workers=a.workers)
+    split = a.split if a.split > 0 else None
+    nseq = min(a.seq_n, len(reqs))
+    log("serve: sequential FP64 solves of the first %d requests" % nseq)
+    seq, seq_wall = batch.run_sequential([(m, b) for _, m, b in reqs[:nseq]])
+    log("serve: sequential %.1f s" % seq_wall)
+    srv = batch.Server(workers=a.workers, split=split)
+    peak = [0]
+
     def progress(done, total, el):
+        u = _device_used_bytes()
+        if u:
+            peak[0] = max(peak[0], u)
         if done % 50 == 0 or done == total:
             log("serve: %d/%d requests done, %.1f s" % (done, total, el))
     res, wall = srv.run(reqs, progress=progress)
     lat = batch.latency_summary(res)
     ok = [r for r in res if not r.get("error")]
+    rate = len(ok) / wall
+    seq_rate = nseq / seq_wall
+    dE, ndE = _energy_check(seq, res[:nseq])
     out = {
         "metric": "serving RHF p99 latency, Poisson %.0f/s chain molecules STO-3G/6-31G(d)" % a.rate,
         "value": round(lat["p99_s"], 4), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": round(1e3 * wall, 1), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic: Poisson arrivals, seed 7; chain molecules 5-120 atoms log-uniform",
+        "dtype": "f64+f32" if split else "f64",
+        "data": "synthetic: Poisson arrivals, seed 7; chain molecules 5-120 atoms log-uniform",
         "config": {"workload": "serving loop, full RHF per request (atomic-density guess, thr 1e-10)",
-                   "rate_per_s": a.rate, "duration_s": a.duration, "workers": a.workers,
+                   "rate_per_s": a.rate, "duration_s": a.duration, "workers": a.workers, "mixed_split": split,
                    "parallelism": "single GPU"},
         "requests": len(reqs), "completed": len(ok), "converged": sum(r.get("converged", False) for r in res),
         "p50_s": round(lat["p50_s"], 4), "p99_s": round(lat["p99_s"], 4), "mean_s": round(lat["mean_s"], 4),
-        "max_s": round(lat["max_s"], 4), "solves_per_s": round(len(ok) / wall, 3), "wall_s": round(wall, 2),
+        "max_s": round(lat["max_s"], 4), "solves_per_s": round(rate, 3),
+        "sequential_fp64_solves_per_s": round(seq_rate, 4),
+        "sequential_sample": "first %d requests, one at a time, FP64" % nseq,
+        "vs_sequential": round(rate / seq_rate, 3),
+        "energy_error_max_Eh": dE, "energy_error_jobs": ndE,
+        "peak_device_bytes": peak[0] or None,
+        "wall_s": round(wall, 2),
     }
     print(json.dumps(out), flush=True)
 
@@ -188,6 +248,7 @@ def main():
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--seq-n", type=int, default=32, help="batch/serve: jobs of the sequential FP64 baseline")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

@@ -55,8 +55,11 @@ def memory_estimate(st):
 DIIS_START = float("inf")   # DIIS from the second iteration on
 
 
-def solve(mol, basis_name, maxiter=50, thr=scf.INTTHRESH, diis_start=DIIS_START):
+def solve(mol, basis_name, maxiter=50, thr=scf.INTTHRESH, diis_start=DIIS_START, split=None):
     """One RHF solve on the GPU; returns a result dict.
+
+    `split`: quartets whose Schwarz x density bound is below it run in FP32
+    (FockBuilder.set_mixed); None = all FP64.
 
     Guess: the superposition of the basis file's atomic densities where the
     basis has them (STO-3G, 6-31G(d): the reference's USEDENSITYSUPGUESS,
@@ -73,6 +76,8 @@ def solve(mol, basis_name, maxiter=50, thr=scf.INTTHRESH, diis_start=DIIS_START)
     st = M.ShellTable(mol, basis)
     fb = fock.FockBuilder(st)
     try:
+        if split:
+            fb.set_mixed(split)
         S, T, V = fb.onee(mol.znuc.astype(np.float64), mol.coords_bohr)
         H = T + V
         P0 = M.atomic_density_guess(mol, basis) if basis.get("density") else None
@@ -84,7 +89,7 @@ def solve(mol, basis_name, maxiter=50, thr=scf.INTTHRESH, diis_start=DIIS_START)
         fb.close()
     return dict(name=mol.title, natom=mol.natom, nb=st.nb, basis=basis_name, energy=res.energy,
                 iterations=res.iterations, converged=bool(res.converged), setup_s=t1 - t0, scf_s=t2 - t1,
-                fock_s=res.t_fock)
+                fock_s=res.t_fock, split=split)
 
 
 def _blas_one_thread():
@@ -96,14 +101,14 @@ def _blas_one_thread():
     return threadpool_limits(1)
 
 
-def run_batch(mols, basis_name, workers=16, maxiter=50, progress=None):
+def run_batch(mols, basis_name, workers=16, maxiter=50, progress=None, split=None):
     """Solve every molecule of `mols`; returns (results in input order, wall
     seconds).  `progress(done, total, elapsed_s)` is called as jobs finish."""
     fock.lib()                      # load the library (and initialise the runtime) once
     with _blas_one_thread():
         t0 = time.perf_counter()
         with ThreadPoolExecutor(max_workers=workers) as ex:
-            futs = [ex.submit(solve, m, basis_name, maxiter) for m in mols]
+            futs = [ex.submit(solve, m, basis_name, maxiter, split=split) for m in mols]
             if progress is not None:
                 done = 0
                 for _ in as_completed(futs):
@@ -119,10 +124,11 @@ class Server:
     arrival order when a worker is free and the memory budget allows; the
     latency of a request is completion time - arrival time."""
 
-    def __init__(self, workers=16, mem_budget=200 * (1 << 30), maxiter=50):
+    def __init__(self, workers=16, mem_budget=200 * (1 << 30), maxiter=50, split=None):
         self.workers = workers
         self.mem_budget = mem_budget
         self.maxiter = maxiter
+        self.split = split
         self._lock = threading.Condition()
         self._inflight_mem = 0
         self._inflight = 0
@@ -132,7 +138,7 @@ class Server:
     def _run_one(self, req, t_start, need, out):
         arrival, mol, basis = req
         try:
-            r = solve(mol, basis, self.maxiter)
+            r = solve(mol, basis, self.maxiter, split=self.split)
             r["error"] = None
         except Exception as e:          # a failed job is reported, the loop goes on
             r = dict(name=mol.title, natom=mol.natom, basis=basis, error=repr(e), converged=False)
@@ -175,6 +181,17 @@ class Server:
             wall = time.perf_counter() - t_start
         out.sort(key=lambda r: r["arrival_s"])
         return out, wall
+
+
+def run_sequential(jobs, maxiter=50, split=None):
+    """Solve (molecule, basis) jobs one at a time on one thread (the plain
+    one-job-at-a-time GPU path); returns (results, wall seconds)."""
+    fock.lib()
+    out = []
+    t0 = time.perf_counter()
+    for mol, basis in jobs:
+        out.append(solve(mol, basis, maxiter, split=split))
+    return out, time.perf_counter() - t0
 
 
 def latency_summary(results):
