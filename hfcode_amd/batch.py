@@ -1,13 +1,14 @@
 """Many-molecule jobs on one GPU: batched SCF solves and a serving loop.
 
-Each job is a full RHF solve (``hfcode_amd.scf.rhf``, the restatement of
-direct_hartreefock.f90:98-223) from the core-Hamiltonian guess, with its own
+Each job is a full RHF solve (the driver of direct_hartreefock.f90:98-223,
+on the GPU through ``FockBuilder.scf``, or ``hfcode_amd.scf.rhf`` on the
+host) with its own
 Fock-build context (``FockBuilder``: device pair data, Schwarz tables, work
 lists and one HIP stream).  A pool of host worker threads runs the jobs; the
 library calls release the interpreter lock, so the jobs' Fock builds run
 concurrently on their own streams and fill the GPU that one small molecule
-alone would leave mostly idle.  Host linear algebra (S^-1/2, eigensolves,
-DIIS) runs with one BLAS thread per worker.
+alone would leave mostly idle.  Host linear algebra (the host driver only)
+runs with one BLAS thread per worker.
 
 Admission: a job is started only while the device memory its context would
 take (work-list capacity, integral buffers; estimated from its shell pairs
@@ -55,11 +56,13 @@ def memory_estimate(st):
 DIIS_START = float("inf")   # DIIS from the second iteration on
 
 
-def solve(mol, basis_name, maxiter=50, thr=scf.INTTHRESH, diis_start=DIIS_START, split=None):
+def solve(mol, basis_name, maxiter=50, thr=scf.INTTHRESH, diis_start=DIIS_START, split=None, device_scf=True):
     """One RHF solve on the GPU; returns a result dict.
 
     `split`: quartets whose Schwarz x density bound is below it run in FP32
-    (FockBuilder.set_mixed); None = all FP64.
+    (FockBuilder.set_mixed); None = all FP64.  `device_scf`: the whole SCF
+    iteration on the GPU (FockBuilder.scf, hfx_scf_run); False runs the host
+    driver (scf.rhf) around the GPU builds.
 
     Guess: the superposition of the basis file's atomic densities where the
     basis has them (STO-3G, 6-31G(d): the reference's USEDENSITYSUPGUESS,
@@ -82,8 +85,12 @@ def solve(mol, basis_name, maxiter=50, thr=scf.INTTHRESH, diis_start=DIIS_START,
         H = T + V
         P0 = M.atomic_density_guess(mol, basis) if basis.get("density") else None
         t1 = time.perf_counter()
-        res = scf.rhf(fb, S, H, mol.nocc, mol.nuclear_repulsion(), maxiter=maxiter, intthresh=thr,
-                      diis_start=diis_start, P0=P0)
+        if device_scf:
+            res = fb.scf(S, H, mol.nocc, mol.nuclear_repulsion(), maxiter=maxiter, intthresh=thr,
+                         diis_start=diis_start, P0=P0)
+        else:
+            res = scf.rhf(fb, S, H, mol.nocc, mol.nuclear_repulsion(), maxiter=maxiter, intthresh=thr,
+                          diis_start=diis_start, P0=P0)
         t2 = time.perf_counter()
     finally:
         fb.close()

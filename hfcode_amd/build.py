@@ -3,7 +3,7 @@
 1. regenerate the per-class kernels (hfcode_amd/gen/gen_eri.py) into
    hfcode_amd/csrc/gen/;
 2. compile every .hip translation unit with hipcc for gfx950, in parallel;
-3. link hfcode_amd/libhfx.so.
+3. link hfcode_amd/libhfx.so (with rocBLAS and rocSOLVER for the SCF driver).
 
     python -m hfcode_amd.build [-j N]
 """
@@ -32,7 +32,7 @@ HIPFLAGS = [
 
 
 def sources():
-    srcs = [os.path.join(CSRC, f) for f in ("hfx_kernels.hip", "hfx_host.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("hfx_kernels.hip", "hfx_host.hip", "hfx_scf.hip")]
     srcs += sorted(os.path.join(GEN, f) for f in os.listdir(GEN) if f.endswith(".hip"))
     return srcs
 
@@ -79,7 +79,9 @@ def build(jobs=None, verbose=True):
         objs.append(host_obj)
     if errs:
         raise RuntimeError("hipcc failed:\n" + "\n".join(e[-4000:] for e in errs))
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + sorted(objs)
+    # rocBLAS / rocSOLVER: the products and eigensolves of the SCF driver (hfx_scf.hip)
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + sorted(objs) + [
+        "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
     if verbose:

@@ -543,6 +543,22 @@ int fetch_fock_host(hfx_ctx* c, double* F, int nb) {
   return HFX_OK;
 }
 
+// After an initial-diagonal build without caller tables: the Schwarz values
+// the build wrote out, sorted per pair type, and the screening lists.
+int finish_schwarz(hfx_ctx* c) {
+  int r;
+  if (!c->have_schwarz) {
+    std::vector<double> s(c->totpairs);
+    HIPOK(hipMemcpy(s.data(), c->d_schw_out, s.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
+      for (int k = 0; k < c->npair[t]; ++k) c->schw[t][k] = s[c->offset[t] + k];
+      sort_schwarz(c, t);
+    }
+    if ((r = upload_active(c))) return r;
+  }
+  return HFX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -882,17 +898,30 @@ int hfx_fock_initial_diag(hfx_ctx* c, const double* P, double* F, int nb, double
   if ((r = copy_fock_host(c, P, F, nb))) return r;
   if ((r = build_diag(c, c->d_P, c->d_F, thr, true))) return r;
   if ((r = fetch_fock_host(c, F, nb))) return r;
-  if (!c->have_schwarz) {
-    std::vector<double> s(c->totpairs);
-    HIPOK(hipMemcpy(s.data(), c->d_schw_out, s.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
-      for (int k = 0; k < c->npair[t]; ++k) c->schw[t][k] = s[c->offset[t] + k];
-      sort_schwarz(c, t);
-    }
-    if ((r = upload_active(c))) return r;
+  return finish_schwarz(c);
+}
+
+int hfx_fock_initial_diag_device(hfx_ctx* c, const double* dP, double* dF, double thr, void* stream) {
+  if (!c || !dP || !dF) return HFX_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  if (s && s != c->stream) {
+    hipEvent_t e;
+    HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPOK(hipEventRecord(e, s));
+    HIPOK(hipStreamWaitEvent(c->stream, e, 0));
+    hipEventDestroy(e);
   }
+  int r = build_diag(c, dP, dF, thr, true);
+  if (r) return r;
+  if (!c->have_schwarz) HIPOK(hipStreamSynchronize(c->stream));
+  if ((r = finish_schwarz(c))) return r;
+  if (s && s != c->stream) HIPOK(hipStreamWaitEvent(s, c->ev1, 0));
   return HFX_OK;
 }
+
+void* hfx_ctx_stream(hfx_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int hfx_ctx_nb(hfx_ctx* c) { return c ? c->nb : HFX_E_ARG; }
 
 int hfx_fock_nd_device(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, void* stream) {
   if (!c || !dP || !dF || (mode != HFX_JK && mode != HFX_J_ONLY)) return HFX_E_ARG;

@@ -15,6 +15,7 @@ libhfx.so raises.
 """
 import ctypes
 import os
+from types import SimpleNamespace
 
 import numpy as np
 
@@ -66,6 +67,18 @@ class Stats(ctypes.Structure):
     ]
 
 
+class ScfOptions(ctypes.Structure):
+    _fields_ = [("maxiter", ctypes.c_int), ("ndiis", ctypes.c_int), ("conv_e", ctypes.c_double),
+                ("conv_p", ctypes.c_double), ("lowthresh", ctypes.c_double), ("intthresh", ctypes.c_double),
+                ("fdifffac", ctypes.c_double), ("diis_start", ctypes.c_double), ("levelshift", ctypes.c_double),
+                ("mode", ctypes.c_int)]
+
+
+class ScfResult(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("fock_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -109,6 +122,14 @@ def configure(L):
         getattr(L, n).argtypes = [vp, dp, dp, ctypes.c_int, ctypes.c_double]
     L.hfx_fock_nd_device.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, vp]
     L.hfx_fock_diag_device.argtypes = [vp, vp, vp, ctypes.c_double, vp]
+    L.hfx_fock_initial_diag_device.argtypes = [vp, vp, vp, ctypes.c_double, vp]
+    L.hfx_ctx_stream.argtypes = [vp]
+    L.hfx_ctx_stream.restype = vp
+    L.hfx_ctx_nb.argtypes = [vp]
+    L.hfx_scf_defaults.argtypes = [ctypes.POINTER(ScfOptions)]
+    L.hfx_scf_defaults.restype = None
+    L.hfx_scf_run.argtypes = [vp, dp, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.POINTER(ScfOptions),
+                              ctypes.POINTER(ScfResult), dp, dp, dp, dp]
     L.hfx_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     L.hfx_get_schwarz.argtypes = [vp, ctypes.c_int, dp, ip, ip]
     L.hfx_debug_blocks.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip, dp]
@@ -346,6 +367,37 @@ class FockBuilder:
         m = HFX_J_ONLY if mode == "j" else HFX_JK
         _check(lib().hfx_fock_nd_device(self._h, ctypes.c_void_p(p_ptr), ctypes.c_void_p(f_ptr), float(thr), m,
                                         ctypes.c_void_p(stream or 0)))
+
+    def scf(self, S, H, nocc, unuc, P0=None, levelshift=None, maxiter=25, ndiis=8, conv_e=5e-8, conv_p=1e-5,
+            lowthresh=5e-7, intthresh=1e-10, fdifffac=0.1, diis_start=0.3, mode="jk"):
+        """Direct RHF with every step on the GPU (hfx_scf_run): the driver of
+        direct_hartreefock.f90:98-223 as hfcode_amd.scf.rhf restates it.
+        Returns an object with energy, P, F, eigs, iterations, converged,
+        history [(it, etot, deltae, deltap)], t_fock (s) and t_total (s)."""
+        nb = self.nb
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        H = np.ascontiguousarray(H, dtype=np.float64)
+        if S.shape != (nb, nb) or H.shape != (nb, nb):
+            raise ValueError("S and H must be (%d, %d)" % (nb, nb))
+        p0 = None if P0 is None else np.ascontiguousarray(P0, dtype=np.float64)
+        o = ScfOptions()
+        lib().hfx_scf_defaults(ctypes.byref(o))
+        o.maxiter, o.ndiis, o.conv_e, o.conv_p = int(maxiter), int(ndiis), float(conv_e), float(conv_p)
+        o.lowthresh, o.intthresh, o.fdifffac = float(lowthresh), float(intthresh), float(fdifffac)
+        o.diis_start = min(float(diis_start), 1e300)
+        o.levelshift = float(levelshift or 0.0)
+        o.mode = HFX_J_ONLY if mode == "j" else HFX_JK
+        r = ScfResult()
+        P, F, eigs = np.zeros((nb, nb)), np.zeros((nb, nb)), np.zeros(nb)
+        hist = np.zeros((maxiter, 4))
+        _check(lib().hfx_scf_run(self._h, _dp(S), _dp(H), int(nocc), float(unuc),
+                                 _dp(p0) if p0 is not None else None, ctypes.byref(o), ctypes.byref(r),
+                                 _dp(P), _dp(F), _dp(eigs), _dp(hist)))
+        n = r.iterations
+        history = [(i + 1, hist[i, 0], hist[i, 1], hist[i, 2]) for i in range(n)]
+        return SimpleNamespace(energy=r.energy, P=P, F=F, eigs=eigs, iterations=n, converged=bool(r.converged),
+                               history=history, resmax=hist[:n, 3].copy(), t_fock=r.fock_ms * 1e-3,
+                               t_total=r.total_ms * 1e-3)
 
     def diag_device(self, p_ptr, f_ptr, thr, stream=None):
         _check(lib().hfx_fock_diag_device(self._h, ctypes.c_void_p(p_ptr), ctypes.c_void_p(f_ptr), float(thr),

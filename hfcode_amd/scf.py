@@ -79,7 +79,8 @@ def rhf(builder, S, H, nocc, unuc, maxiter=MAXITER, ndiis=NDIIS, conv_p=CONV_P, 
     if levelshift:
         shft[nocc:] = levelshift
     pold = p.copy()
-    eelec = float(np.sum(p * H))
+    # E of the guess: sum P H for the core guess (direct_hartreefock.f90:83-88), 0 for a given density
+    eelec = float(np.sum(p * H)) if P0 is None else 0.0
     nusediis = 0
     nfull = 0
     faobold = None

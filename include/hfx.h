@@ -105,6 +105,14 @@ int hfx_fock_initial_diag(hfx_ctx* ctx, const double* P, double* F, int nb, doub
  * stream) is waited on before and made to wait after.  No host copies. */
 int hfx_fock_nd_device(hfx_ctx* ctx, const double* dP, double* dF, double thr, int mode, void* stream);
 int hfx_fock_diag_device(hfx_ctx* ctx, const double* dP, double* dF, double thr, void* stream);
+/* fock_build_initial_diag on device arrays; when the context has no Schwarz
+ * tables yet it waits for the build and sorts them on the host. */
+int hfx_fock_initial_diag_device(hfx_ctx* ctx, const double* dP, double* dF, double thr, void* stream);
+
+/* The context's HIP stream (a hipStream_t), for callers that enqueue their own
+ * work in order with the builds, and its number of basis functions. */
+void* hfx_ctx_stream(hfx_ctx* ctx);
+int hfx_ctx_nb(hfx_ctx* ctx);
 
 /* Statistics of the most recent build (blocks until it has finished). */
 int hfx_get_stats(hfx_ctx* ctx, hfx_stats* out);
@@ -158,6 +166,45 @@ int hfx_molecule_atoms(const hfx_molecule* m, double* charges, double* coords);
  * direct_hartreefock.f90:50-64; HFX_E_STATE when the basis file has none (dmat = 0). */
 int hfx_molecule_density_guess(const hfx_molecule* m, double* P);
 void hfx_molecule_free(hfx_molecule* m);
+
+/* Direct RHF on the GPU: the driver loop of direct_hartreefock.f90:98-223
+ * (full, rebuild and incremental dP builds, energy, DIIS error
+ * S^-1/2 (FPS - SPF) S^-1/2, DIIS over up to ndiis Fock matrices with 1.05
+ * damping and the 1.5 coefficient cap, threshold switch, level shift,
+ * convergence test) with S^-1/2 from wrapper_dsyev (direct_hartreefock.f90:23-29)
+ * and the eigensolves of wrapper_dsyev as rocSOLVER dsyevd.  Every matrix
+ * stays on the device on the context's stream; per iteration the host reads
+ * a few scalars (energy, max|resid|, max|dP|, the new DIIS inner products)
+ * and solves the (n+1) x (n+1) DIIS system (wrapper_dsysv).  No counterpart
+ * entry point in the reference: it replaces the driver, not a routine. */
+typedef struct hfx_scf_options {
+  int maxiter;          /* maxiter (25) */
+  int ndiis;            /* ndiis (8), at most 16 */
+  double conv_e;        /* conve (5e-8) */
+  double conv_p;        /* conv (1e-5) */
+  double lowthresh;     /* lowthresh (5e-7) */
+  double intthresh;     /* intthresh (1e-10) */
+  double fdifffac;      /* fdifffac (0.1) */
+  double diis_start;    /* DIIS starts when max|resid| < this (0.3) */
+  double levelshift;    /* lshft; 0 = no level shift (levshft false) */
+  int mode;             /* HFX_JK */
+} hfx_scf_options;
+typedef struct hfx_scf_result {
+  double energy;        /* etot of the last iteration */
+  int iterations;
+  int converged;        /* 1 when |dE| < conv_e and max|dP| < conv_p */
+  double fock_ms;       /* device time of the Fock builds */
+  double total_ms;      /* wall time of hfx_scf_run */
+} hfx_scf_result;
+void hfx_scf_defaults(hfx_scf_options* opt);
+/* S, H: nb*nb host arrays (H = T + V).  P0: starting density (nb*nb, the
+ * atomic-density guess) or NULL for the core-Hamiltonian guess.  Outputs
+ * (each may be NULL): P the density of the last build, F its Fock matrix,
+ * eigs the final orbital energies (nb), history 4 doubles per iteration
+ * (etot, deltae, deltap, max|resid|; maxiter rows). */
+int hfx_scf_run(hfx_ctx* ctx, const double* S, const double* H, int nocc, double unuc, const double* P0,
+                const hfx_scf_options* opt, hfx_scf_result* res, double* P, double* F, double* eigs,
+                double* history);
 
 const char* hfx_version(void);
 

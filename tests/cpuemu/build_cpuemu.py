@@ -23,7 +23,8 @@ FLAGS = ["-x", "c++", "-std=c++17", "-O1", "-fPIC", "-w", "-I" + HERE, "-I" + os
 
 def _one(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    deps = [src, os.path.join(HERE, "hip", "hip_runtime.h"), os.path.join(CSRC, "hfx_device.h"),
+    deps = [src, os.path.join(HERE, "hip", "hip_runtime.h"), os.path.join(HERE, "rocblas", "rocblas.h"),
+            os.path.join(HERE, "rocsolver", "rocsolver.h"), os.path.join(CSRC, "hfx_device.h"),
             os.path.join(CSRC, "hfx_kernels.h"), os.path.join(GEN, "eri_dispatch.h")]
     if os.path.exists(obj) and all(os.path.getmtime(obj) >= os.path.getmtime(d) for d in deps):
         return None
@@ -38,7 +39,7 @@ def build(jobs=8):
     gen_eri.main(GEN)
     gen_onee.main(GEN)
     os.makedirs(OBJ, exist_ok=True)
-    srcs = [os.path.join(CSRC, "hfx_kernels.hip"), os.path.join(CSRC, "hfx_host.hip"),
+    srcs = [os.path.join(CSRC, "hfx_kernels.hip"), os.path.join(CSRC, "hfx_host.hip"), os.path.join(CSRC, "hfx_scf.hip"),
             os.path.join(HERE, "emu_smem.cpp"), os.path.join(CSRC, "hfx_onee.cpp"),
             os.path.join(CSRC, "hfx_input.cpp")]
     srcs += sorted(os.path.join(GEN, f) for f in os.listdir(GEN) if f.endswith(".hip"))

@@ -102,6 +102,10 @@ inline hipError_t hipMalloc(void** p, size_t n) {
   *p = std::calloc(1, n);
   return *p ? hipSuccess : 2;
 }
+template <typename T>
+inline hipError_t hipMalloc(T** p, size_t n) {
+  return hipMalloc((void**)p, n);
+}
 inline hipError_t hipFree(void* p) {
   std::free(p);
   return hipSuccess;
@@ -112,6 +116,10 @@ inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
 }
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) {
   return hipMemcpy(d, s, n, k);
+}
+inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) {
+  std::memset(d, v, n);
+  return hipSuccess;
 }
 inline hipError_t hipGetDevice(int* d) {
   *d = 0;

@@ -100,6 +100,38 @@ def test_standalone_path_and_scf(emu_fock, name):
     assert res.converged and abs(res.energy - float(g["E_scf"])) < 1e-9
 
 
+@pytest.mark.parametrize("name,basis,ls", [("water_sto3g", "sto-3g", None), ("water_def2svp", "def2-sv(p)", None),
+                                            ("benzene_631gd", "6-31g(d)", 0.4)])
+def test_device_scf_driver(emu_fock, name, basis, ls):
+    """hfx_scf_run (the SCF driver's device code, with serial stand-ins for
+    dgemm and dsyevd) against the reference's converged energy and the host
+    restatement of the driver, iteration by iteration."""
+    g = Golden(name)
+    mol = molecule(name)
+    st = M.ShellTable(mol, M.load_basis(basis))
+    fb = emu_fock.FockBuilder(st)
+    S, T, V = fb.onee(mol.znuc.astype(np.float64), mol.coords_bohr)
+    dev = fb.scf(S, T + V, mol.nocc, mol.nuclear_repulsion(), levelshift=ls)
+    host = scf.rhf(emu_fock.FockBuilder(st), S, T + V, mol.nocc, mol.nuclear_repulsion(), levelshift=ls)
+    assert dev.converged and abs(dev.energy - float(g["E_scf"])) < 1e-9
+    assert dev.iterations == host.iterations
+    for a, b in zip(dev.history, host.history):
+        assert abs(a[1] - b[1]) < 1e-9 and abs(a[3] - b[3]) < 1e-8, a[0]
+    assert np.abs(dev.P - host.P).max() < 1e-9 and np.abs(dev.F - host.F).max() < 1e-9
+
+
+def test_device_scf_driver_arguments(emu_fock):
+    g = Golden("water_sto3g")
+    fb = emu_fock.FockBuilder(g.shells, nb=g.nb)
+    S = np.eye(g.nb)
+    with pytest.raises(emu_fock.HfxError, match="-1"):
+        fb.scf(S, S, 0, 0.0)                       # no occupied orbitals
+    with pytest.raises(emu_fock.HfxError, match="-1"):
+        fb.scf(S, S, 5, 0.0, ndiis=17)             # more DIIS vectors than the driver keeps
+    with pytest.raises(ValueError):
+        fb.scf(np.eye(3), S, 5, 0.0)
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_device_one_electron_matches_reference(emu_fock, name):
     """Generated S/T/V kernels (hfx_ctx_onee) against the reference's S, T, V

@@ -1,0 +1,64 @@
+"""The SCF driver on the GPU (hfx_scf_run: S^-1/2, builds, energy, DIIS,
+rocSOLVER eigensolves, density, all device-resident) against the converged
+energies of the reference driver with the reference's Fock builds (golden
+E_scf) and against the host restatement of the same driver
+(hfcode_amd.scf.rhf) iteration by iteration."""
+import numpy as np
+import pytest
+
+from hfcode_amd import batch, fock, molecule as M, scf
+from tests.golden_io import Golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name,mol,basis,ls", [("water_sto3g", M.water, "sto-3g", None),
+                                              ("benzene_631gd", M.benzene, "6-31g(d)", 0.4),
+                                              ("water_def2svp", M.water, "def2-sv(p)", None)])
+def test_device_driver_matches_reference_energy(name, mol, basis, ls):
+    g = Golden(name)
+    m = mol()
+    st = M.ShellTable(m, M.load_basis(basis))
+    fb = fock.FockBuilder(st)
+    S, T, V = fb.onee(m.znuc.astype(np.float64), m.coords_bohr)
+    dev = fb.scf(S, T + V, m.nocc, m.nuclear_repulsion(), levelshift=ls)
+    host = scf.rhf(fock.FockBuilder(st), S, T + V, m.nocc, m.nuclear_repulsion(), levelshift=ls)
+    print(name, dev.energy - float(g["E_scf"]), dev.iterations, host.iterations, dev.t_total, flush=True)
+    assert dev.converged and abs(dev.energy - float(g["E_scf"])) < 1e-9
+    assert dev.iterations == host.iterations
+    for a, b in zip(dev.history, host.history):
+        assert abs(a[1] - b[1]) < 1e-9, a[0]
+    assert np.abs(dev.P - host.P).max() < 1e-8
+    assert np.abs(dev.eigs - host.eigs).max() < 1e-8
+    fb.close()
+
+
+def test_device_driver_density_guess_and_batch_settings():
+    """Atomic-density guess (USEDENSITYSUPGUESS) and the batch jobs' DIIS
+    from the second iteration, against the host driver."""
+    m = M.benzene()
+    bas = M.load_basis("6-31g(d)")
+    st = M.ShellTable(m, bas)
+    fb = fock.FockBuilder(st)
+    S, T, V = fb.onee(m.znuc.astype(np.float64), m.coords_bohr)
+    P0 = M.atomic_density_guess(m, bas)
+    dev = fb.scf(S, T + V, m.nocc, m.nuclear_repulsion(), P0=P0)
+    host = scf.rhf(fock.FockBuilder(st), S, T + V, m.nocc, m.nuclear_repulsion(), P0=P0)
+    assert dev.converged and dev.iterations == host.iterations and abs(dev.energy - host.energy) < 1e-9
+    c = M.chain(8, "alkanol", jitter=0.05, rng=np.random.default_rng(3))
+    st = M.ShellTable(c, M.load_basis("def2-sv(p)"))
+    fb = fock.FockBuilder(st)
+    S, T, V = fb.onee(c.znuc.astype(np.float64), c.coords_bohr)
+    dev = fb.scf(S, T + V, c.nocc, c.nuclear_repulsion(), maxiter=50, diis_start=float("inf"))
+    host = scf.rhf(fock.FockBuilder(st), S, T + V, c.nocc, c.nuclear_repulsion(), maxiter=50,
+                   diis_start=float("inf"))
+    print(c.title, dev.energy, host.energy, dev.iterations, host.iterations, flush=True)
+    assert dev.converged and dev.iterations == host.iterations and abs(dev.energy - host.energy) < 1e-9
+
+
+def test_batch_jobs_with_device_driver():
+    mols = [M.chain(n, k, jitter=0.05, rng=np.random.default_rng(n)) for n, k in ((5, "alkane"), (4, "amine"))]
+    host = [batch.solve(m, "def2-sv(p)", device_scf=False) for m in mols]
+    dev, wall = batch.run_batch(mols, "def2-sv(p)", workers=2)
+    for h, d in zip(host, dev):
+        assert d["converged"] and abs(d["energy"] - h["energy"]) < 1e-9
