@@ -156,7 +156,8 @@ def run_batch(a):
             peak[0] = max(peak[0], u)
         if done % 16 == 0 or done == total:
             log("batch: %d/%d solved, %.1f s" % (done, total, el))
-    res, wall = batch.run_batch(mols, "def2-sv(p)", workers=a.workers, progress=progress, split=split)
+    res, wall = batch.run_batch(mols, "def2-sv(p)", workers=a.workers, progress=progress, split=split,
+                                device_scf=a.scf == "device")
     nbs = [r["nb"] for r in res]
     its = [r["iterations"] for r in res]
     conv = sum(r["converged"] for r in res)
@@ -173,7 +174,7 @@ def run_batch(a):
         "config": {"workload": "batched RHF solves (core guess, DIIS from iteration 2, thr 1e-10, <= 50 iterations)",
                    "basis": "def2-SV(P)", "molecules": len(mols), "nb_min": min(nbs), "nb_max": max(nbs),
                    "nb_mean": round(float(np.mean(nbs)), 1), "workers": a.workers, "mixed_split": split,
-                   "parallelism": "single GPU"},
+                   "scf_driver": a.scf, "parallelism": "single GPU"},
         "converged": conv, "iterations_mean": round(float(np.mean(its)), 2), "iterations_max": max(its),
         "solve_s_p50": round(float(np.median([r["setup_s"] + r["scf_s"] for r in res])), 4),
         "fock_s_fraction": round(float(sum(r["fock_s"] for r in res) / sum(r["setup_s"] + r["scf_s"] for r in res)), 3),
@@ -195,7 +196,7 @@ def run_serve(a):
     log("serve: sequential FP64 solves of the first %d requests" % nseq)
     seq, seq_wall = batch.run_sequential([(m, b) for _, m, b in reqs[:nseq]])
     log("serve: sequential %.1f s" % seq_wall)
-    srv = batch.Server(workers=a.workers, split=split)
+    srv = batch.Server(workers=a.workers, split=split, device_scf=a.scf == "device")
     peak = [0]
 
     def progress(done, total, el):
@@ -218,7 +219,7 @@ def run_serve(a):
         "data": "synthetic: Poisson arrivals, seed 7; chain molecules 5-120 atoms log-uniform",
         "config": {"workload": "serving loop, full RHF per request (atomic-density guess, thr 1e-10)",
                    "rate_per_s": a.rate, "duration_s": a.duration, "workers": a.workers, "mixed_split": split,
-                   "parallelism": "single GPU"},
+                   "scf_driver": a.scf, "parallelism": "single GPU"},
         "requests": len(reqs), "completed": len(ok), "converged": sum(r.get("converged", False) for r in res),
         "p50_s": round(lat["p50_s"], 4), "p99_s": round(lat["p99_s"], 4), "mean_s": round(lat["mean_s"], 4),
         "max_s": round(lat["max_s"], 4), "solves_per_s": round(rate, 3),
@@ -249,6 +250,8 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--seq-n", type=int, default=32, help="batch/serve: jobs of the sequential FP64 baseline")
+    ap.add_argument("--scf", choices=["host", "device"], default="host",
+                    help="batch/serve: SCF driver of each job (host: scf.rhf, device: hfx_scf_run)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

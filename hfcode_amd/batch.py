@@ -1,14 +1,14 @@
 """Many-molecule jobs on one GPU: batched SCF solves and a serving loop.
 
-Each job is a full RHF solve (the driver of direct_hartreefock.f90:98-223,
-on the GPU through ``FockBuilder.scf``, or ``hfcode_amd.scf.rhf`` on the
-host) with its own
+Each job is a full RHF solve (the driver of direct_hartreefock.f90:98-223:
+``hfcode_amd.scf.rhf`` on the host around the GPU builds, or with
+``device_scf`` all of it on the GPU through ``FockBuilder.scf``) with its own
 Fock-build context (``FockBuilder``: device pair data, Schwarz tables, work
 lists and one HIP stream).  A pool of host worker threads runs the jobs; the
 library calls release the interpreter lock, so the jobs' Fock builds run
 concurrently on their own streams and fill the GPU that one small molecule
-alone would leave mostly idle.  Host linear algebra (the host driver only)
-runs with one BLAS thread per worker.
+alone would leave mostly idle.  Host linear algebra runs with one BLAS
+thread per worker.
 
 Admission: a job is started only while the device memory its context would
 take (work-list capacity, integral buffers; estimated from its shell pairs
@@ -56,13 +56,17 @@ def memory_estimate(st):
 DIIS_START = float("inf")   # DIIS from the second iteration on
 
 
-def solve(mol, basis_name, maxiter=50, thr=scf.INTTHRESH, diis_start=DIIS_START, split=None, device_scf=True):
+def solve(mol, basis_name, maxiter=50, thr=scf.INTTHRESH, diis_start=DIIS_START, split=None,
+          device_scf=False):
     """One RHF solve on the GPU; returns a result dict.
 
     `split`: quartets whose Schwarz x density bound is below it run in FP32
     (FockBuilder.set_mixed); None = all FP64.  `device_scf`: the whole SCF
-    iteration on the GPU (FockBuilder.scf, hfx_scf_run); False runs the host
-    driver (scf.rhf) around the GPU builds.
+    iteration on the GPU (FockBuilder.scf, hfx_scf_run); the default runs the
+    host driver (scf.rhf) around the GPU builds, which is faster when many
+    small jobs share the GPU (measured: 512 def2-SV(P) chains 3.61 solves/s
+    host driver, 2.99 device driver; rocSOLVER's dsyevd at nb ~200 is many
+    small launches, which compete with the other jobs' builds).
 
     Guess: the superposition of the basis file's atomic densities where the
     basis has them (STO-3G, 6-31G(d): the reference's USEDENSITYSUPGUESS,
@@ -108,14 +112,14 @@ def _blas_one_thread():
     return threadpool_limits(1)
 
 
-def run_batch(mols, basis_name, workers=16, maxiter=50, progress=None, split=None):
+def run_batch(mols, basis_name, workers=16, maxiter=50, progress=None, split=None, device_scf=False):
     """Solve every molecule of `mols`; returns (results in input order, wall
     seconds).  `progress(done, total, elapsed_s)` is called as jobs finish."""
     fock.lib()                      # load the library (and initialise the runtime) once
     with _blas_one_thread():
         t0 = time.perf_counter()
         with ThreadPoolExecutor(max_workers=workers) as ex:
-            futs = [ex.submit(solve, m, basis_name, maxiter, split=split) for m in mols]
+            futs = [ex.submit(solve, m, basis_name, maxiter, split=split, device_scf=device_scf) for m in mols]
             if progress is not None:
                 done = 0
                 for _ in as_completed(futs):
@@ -131,11 +135,12 @@ class Server:
     arrival order when a worker is free and the memory budget allows; the
     latency of a request is completion time - arrival time."""
 
-    def __init__(self, workers=16, mem_budget=200 * (1 << 30), maxiter=50, split=None):
+    def __init__(self, workers=16, mem_budget=200 * (1 << 30), maxiter=50, split=None, device_scf=False):
         self.workers = workers
         self.mem_budget = mem_budget
         self.maxiter = maxiter
         self.split = split
+        self.device_scf = device_scf
         self._lock = threading.Condition()
         self._inflight_mem = 0
         self._inflight = 0
@@ -145,7 +150,7 @@ class Server:
     def _run_one(self, req, t_start, need, out):
         arrival, mol, basis = req
         try:
-            r = solve(mol, basis, self.maxiter, split=self.split)
+            r = solve(mol, basis, self.maxiter, split=self.split, device_scf=self.device_scf)
             r["error"] = None
         except Exception as e:          # a failed job is reported, the loop goes on
             r = dict(name=mol.title, natom=mol.natom, basis=basis, error=repr(e), converged=False)

@@ -58,7 +58,7 @@ def test_device_driver_density_guess_and_batch_settings():
 
 def test_batch_jobs_with_device_driver():
     mols = [M.chain(n, k, jitter=0.05, rng=np.random.default_rng(n)) for n, k in ((5, "alkane"), (4, "amine"))]
-    host = [batch.solve(m, "def2-sv(p)", device_scf=False) for m in mols]
-    dev, wall = batch.run_batch(mols, "def2-sv(p)", workers=2)
+    host = [batch.solve(m, "def2-sv(p)") for m in mols]
+    dev, wall = batch.run_batch(mols, "def2-sv(p)", workers=2, device_scf=True)
     for h, d in zip(host, dev):
         assert d["converged"] and abs(d["energy"] - h["energy"]) < 1e-9
