@@ -61,6 +61,8 @@ struct ClassArgs {
   int diag;                // 0 off-diagonal build, 1 bra==ket build
   int q0, q1;              // block kernels: chunk of the quartet list
   int vstride;             // block kernels: row length of vmax
+  int lds_rows;            // fused kernels: 1 = rows of X of the resident bra pair in LDS
+  int rows_offset;         // fused kernels: byte offset of those rows in the dynamic LDS
 };
 
 // Compiler barrier between slices of an unrolled load sequence: keeps the

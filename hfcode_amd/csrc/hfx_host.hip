@@ -243,8 +243,17 @@ int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
   const void* fn = prec == 1 ? k.f32 : (jonly ? k.f64j : k.f64);
   const int K = prec == 1 ? 5 : 8;
-  const size_t shm = (size_t)HFX_BOYS_ROWS * (k.M + K) * (prec == 1 ? sizeof(float) : sizeof(double));
-  const int grid = grid_for(cap, HFX_NT, c->ncu * 8);
+  const size_t boys_shm = (size_t)HFX_BOYS_ROWS * (k.M + K) * (prec == 1 ? sizeof(float) : sizeof(double));
+  // rows of X of the resident bra pair (shells A and B) after the Boys table,
+  // where at least two workgroups per CU still fit (off in the diagonal builds)
+  const size_t rows_off = (boys_shm + 15) / 16 * 16;
+  const size_t rows_shm = (size_t)(k.dims[0] + k.dims[1]) * c->nb * sizeof(double);
+  const bool rows = !a.diag && rows_off + rows_shm <= c->max_shm / 2;
+  const size_t shm = rows ? rows_off + rows_shm : boys_shm;
+  a.lds_rows = rows ? 1 : 0;
+  a.rows_offset = (int)rows_off;
+  const int per_cu = rows ? (int)std::max<size_t>(2, std::min<size_t>(8, c->max_shm / shm)) : 8;
+  const int grid = grid_for(cap, HFX_NT, c->ncu * per_cu);
   void* args[] = {&a};
   HIPOK(hipLaunchKernel(fn, dim3(grid), dim3(HFX_NT), args, shm, c->stream));
   if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));

@@ -558,9 +558,19 @@ class Gen:
 
     def kernel_fused(self, W, ops_prim, ops_bra, ops_ctr):
         """One thread per quartet; integrals, block test and J/K digestion in
-        registers (light classes)."""
+        registers (light classes).  With a.lds_rows the workgroup walks a
+        contiguous range of the bra-contiguous list, 256 quartets per step,
+        and keeps the rows of X of one bra pair (the bra of the step's last
+        quartet) in LDS after the Boys table: J_ab (summed over the wave
+        first), K_ac, K_ad, K_bc and K_bd of that bra's quartets add there and
+        go to X with row-coalesced atomics when the bra changes and at the
+        end; J_cd and other quartets add to X directly.  Without it (diagonal
+        builds, rows larger than the LDS allows) every sum is an atomic to X."""
         cls = self.cls
         M = self.M
+        nA, nB, nC, nD = self.dims()
+        oa = "b" if self.swap_bra else "a"
+        ob = "a" if self.swap_bra else "b"
         W("template <typename R, bool JONLY>")
         W("__global__ void __launch_bounds__(HFX_NT) eri_%s(const ClassArgs a) {" % cls)
         W("  extern __shared__ char hfx_smem[];")
@@ -569,13 +579,53 @@ class Gen:
         W("  stage_boys<R>(btab, a, NB_COL);")
         W("  const int nq = *a.count;")
         W("  const int nbf = a.nb;")
-        W("  for (int iq = blockIdx.x * blockDim.x + threadIdx.x; iq < nq; iq += gridDim.x * blockDim.x) {")
+        W("  const bool rows = a.lds_rows != 0;")
+        W("  double* srow = reinterpret_cast<double*>(hfx_smem + a.rows_offset);   // [%d][nb]: rows A then B" % (nA + nB))
+        W("  const int lane = threadIdx.x & (HFX_WAVE - 1);")
+        W("  double* X = a.X;")
+        W("  const int nsteps = (nq + blockDim.x - 1) / blockDim.x;")
+        W("  int s0, s1;")
+        W("  if (rows) {   // contiguous range of steps per workgroup")
+        W("    const int per = (nsteps + gridDim.x - 1) / gridDim.x;")
+        W("    s0 = blockIdx.x * per; s1 = min(nsteps, s0 + per);")
+        W("    for (int e = threadIdx.x; e < %d * nbf; e += blockDim.x) srow[e] = 0.0;" % (nA + nB))
+        W("  } else {")
+        W("    s0 = blockIdx.x; s1 = nsteps;")
+        W("  }")
+        W("  __syncthreads();   // rows zeroed before any wave adds to them")
+        W("  const int sstride = rows ? 1 : gridDim.x;")
+        W("  int curb = -1, rA = 0, rB = 0;   // resident bra pair and its rows")
+        W("  auto flush = [&]() {")
+        W("    __syncthreads();")
+        W("    for (int e = threadIdx.x; e < %d * nbf; e += blockDim.x) {" % (nA + nB))
+        W("      const double v = srow[e];")
+        W("      if (v != 0.0) {")
+        W("        const int r = e / nbf, col = e - r * nbf;")
+        W("        const int row = r < %d ? rA + r : rB + (r - %d);" % (nA, nA))
+        W("        atomicAdd(&X[(size_t)row * nbf + col], v);")
+        W("        srow[e] = 0.0;")
+        W("      }")
+        W("    }")
+        W("    __syncthreads();")
+        W("  };")
+        W("  for (int st = s0; st < s1; st += sstride) {")
+        W("    if (rows) {")
+        W("      const int tb = a.list[min((st + 1) * (int)blockDim.x, nq) - 1].x;")
+        W("      if (tb != curb) {")
+        W("        if (curb >= 0) flush();")
+        W("        curb = tb;")
+        W("        rA = a.pairs[tb].ao%s; rB = a.pairs[tb].ao%s;" % (oa, ob))
+        W("      }")
+        W("    }")
+        W("    const int it = st * blockDim.x + (threadIdx.x - lane);   // first quartet of the wave")
+        W("    if (it >= nq) continue;")
+        W("    const bool valid = it + lane < nq;")
+        W("    const int iq = valid ? it + lane : nq - 1;")
         W("    const int2 bk = a.list[iq];")
         self.emit_quartet_setup(W)
         elem = {}
         for gi, combos in enumerate(self.group_plan(False)):
             elem.update(self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%d" % gi))
-        nA, nB, nC, nD = self.dims()
         assert len(elem) == nA * nB * nC * nD
 
         def pblk(name, ao1, n1, ao2, n2, konly):
@@ -602,7 +652,7 @@ class Gen:
         decl("Kac", nA, nC); decl("Kad", nA, nD); decl("Kbc", nB, nC); decl("Kbd", nB, nD)
         W("    R vmax = 0;")
         W("    const R ecut = a.diag ? (R)HFX_INTTHRESH : (R)0;")
-        W("    const bool dump = a.blocks != nullptr;")
+        W("    const bool dump = a.blocks != nullptr && valid;")
         for (ia, ib, ic, id_), nm in sorted(elem.items()):
             W("    {")
             W("      R v = %s;" % nm)
@@ -620,26 +670,48 @@ class Gen:
             W("      }")
             W("    }")
             ops_ctr.n += 12 + 1
-        W("    if (a.schw_out) a.schw_out[bk.x] = sqrt((double)vmax);")
-        W("    if (!a.diag && (double)vmax < a.thr) continue;")
-        W("    const double w = ((shA == shB) ? 0.5 : 1.0) * ((shC == shD) ? 0.5 : 1.0) * (a.diag ? 0.5 : 1.0);")
+        W("    if (a.schw_out && valid) a.schw_out[bk.x] = sqrt((double)vmax);")
+        W("    const bool act = valid && (a.diag || (double)vmax >= a.thr);")
+        W("    if (__ballot(act) == 0) continue;")
+        W("    const bool inl = rows && act && bk.x == curb;   // rows A, B resident in LDS")
+        W("    const double w = act ? ((shA == shB) ? 0.5 : 1.0) * ((shC == shD) ? 0.5 : 1.0) * (a.diag ? 0.5 : 1.0) : 0.0;")
         W("    const double wj = 2.0 * w, wk = -0.5 * w;")
-        W("    double* X = a.X;")
+        for i in range(nA):
+            for j in range(nB):
+                W("    {")
+                W("      const double jab = wj * Jab_%d_%d;" % (i, j))
+                W("      if (rows) {")
+                W("        double r = inl ? jab : 0.0;")
+                W("        for (int o = HFX_WAVE / 2; o > 0; o >>= 1) r += __shfl_xor(r, o);")
+                W("        if (lane == 0 && r != 0.0) atomicAdd(&srow[%d * nbf + rB + %d], r);" % (i, j))
+                W("      }")
+                W("      if (act && !inl) atomicAdd(&X[(size_t)(aoA + %d) * nbf + aoB + %d], jab);" % (i, j))
+                W("    }")
+        W("    if (act) {")
+        for i in range(nC):
+            for j in range(nD):
+                W("      atomicAdd(&X[(size_t)(aoC + %d) * nbf + aoD + %d], wj * Jcd_%d_%d);" % (i, j, i, j))
+        W("    }")
 
-        def flush(name, ao1, n1, ao2, n2, wname):
+        def krow(name, ao1, n1, row0, ao2, n2):
+            W("      if (inl) {")
             for i in range(n1):
                 for j in range(n2):
-                    W("    atomicAdd(&X[(size_t)(%s + %d) * nbf + %s + %d], %s * %s_%d_%d);" % (ao1, i, ao2, j, wname, name, i, j))
+                    W("        atomicAdd(&srow[%d * nbf + %s + %d], wk * %s_%d_%d);" % (row0 + i, ao2, j, name, i, j))
+            W("      } else if (act) {")
+            for i in range(n1):
+                for j in range(n2):
+                    W("        atomicAdd(&X[(size_t)(%s + %d) * nbf + %s + %d], wk * %s_%d_%d);" % (ao1, i, ao2, j, name, i, j))
+            W("      }")
 
-        flush("Jab", "aoA", nA, "aoB", nB, "wj")
-        flush("Jcd", "aoC", nC, "aoD", nD, "wj")
         W("    if constexpr (!JONLY) {")
-        flush("Kac", "aoA", nA, "aoC", nC, "wk")
-        flush("Kad", "aoA", nA, "aoD", nD, "wk")
-        flush("Kbc", "aoB", nB, "aoC", nC, "wk")
-        flush("Kbd", "aoB", nB, "aoD", nD, "wk")
+        krow("Kac", "aoA", nA, 0, "aoC", nC)
+        krow("Kad", "aoA", nA, 0, "aoD", nD)
+        krow("Kbc", "aoB", nB, nA, "aoC", nC)
+        krow("Kbd", "aoB", nB, nA, "aoD", nD)
         W("    }")
         W("  }")
+        W("  if (rows && curb >= 0) flush();")
         W("}")
 
     def combo_M(self, combo):
@@ -756,6 +828,7 @@ class Gen:
         W("  } else {")
         W("    s0 = blockIdx.x; s1 = nsteps;")
         W("  }")
+        W("  __syncthreads();   // rows zeroed before any wave adds to them")
         W("  const int sstride = rows ? 1 : gridDim.x;")
         W("  int curb = -1, rA = 0, rB = 0;   // resident bra pair and its rows")
         W("  auto flush = [&]() {")
