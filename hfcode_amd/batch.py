@@ -98,9 +98,10 @@ def solve(mol, basis_name, maxiter=50, thr=scf.INTTHRESH, diis_start=DIIS_START,
         t2 = time.perf_counter()
     finally:
         fb.close()
+    t3 = time.perf_counter()
     return dict(name=mol.title, natom=mol.natom, nb=st.nb, basis=basis_name, energy=res.energy,
                 iterations=res.iterations, converged=bool(res.converged), setup_s=t1 - t0, scf_s=t2 - t1,
-                fock_s=res.t_fock, split=split)
+                close_s=t3 - t2, fock_s=res.t_fock, split=split)
 
 
 def _blas_one_thread():

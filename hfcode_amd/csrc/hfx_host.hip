@@ -6,7 +6,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/hfx.h"
@@ -75,11 +78,102 @@ void boys_table(std::vector<double>& t64, std::vector<float>& t32) {
   }
 }
 
+// Device memory of the contexts comes from a per-process cache of freed
+// blocks (size classes, four per power of two, per device): hipFree waits
+// for the whole device, so a job that ends while others are running would
+// stall its thread until all their queued work had drained.  A block is
+// returned to the cache only when its context's stream no longer uses it;
+// hfx_release_cached_memory hands the cached blocks back to HIP.
+struct BlockCache {
+  std::mutex mu;
+  std::multimap<std::pair<int, size_t>, void*> free_blocks;   // (device, bytes) -> block
+  std::unordered_map<void*, std::pair<int, size_t>> live;
+};
+BlockCache& block_cache() {
+  static BlockCache* bc = new BlockCache;   // lives until the process ends
+  return *bc;
+}
+
+size_t size_class(size_t n) {
+  if (n <= 4096) return 4096;
+  const int lg = 63 - __builtin_clzll((unsigned long long)(n - 1));
+  const size_t step = (size_t)1 << (lg - 2);
+  return (n + step - 1) / step * step;
+}
+
+void release_cached(int device) {   // device < 0: every device
+  BlockCache& bc = block_cache();
+  std::vector<void*> drop;
+  {
+    std::lock_guard<std::mutex> lk(bc.mu);
+    for (auto it = bc.free_blocks.begin(); it != bc.free_blocks.end();) {
+      if (device < 0 || it->first.first == device) {
+        drop.push_back(it->second);
+        it = bc.free_blocks.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  for (void* p : drop) hipFree(p);
+}
+
+void* dev_alloc(size_t bytes) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  const std::pair<int, size_t> key(dev, size_class(bytes));
+  BlockCache& bc = block_cache();
+  {
+    std::lock_guard<std::mutex> lk(bc.mu);
+    auto it = bc.free_blocks.find(key);
+    if (it != bc.free_blocks.end()) {
+      void* p = it->second;
+      bc.free_blocks.erase(it);
+      bc.live[p] = key;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, key.second) != hipSuccess) {
+    (void)hipGetLastError();
+    release_cached(dev);            // memory held by the cache, then try once more
+    if (hipMalloc(&p, key.second) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
+  std::lock_guard<std::mutex> lk(bc.mu);
+  bc.live[p] = key;
+  return p;
+}
+
+// The caller guarantees that no queued work still uses p.
+void dev_free(void* p) {
+  if (!p) return;
+  BlockCache& bc = block_cache();
+  {
+    std::lock_guard<std::mutex> lk(bc.mu);
+    auto it = bc.live.find(p);
+    if (it != bc.live.end()) {
+      bc.free_blocks.emplace(it->second, p);
+      bc.live.erase(it);
+      return;
+    }
+  }
+  hipFree(p);
+}
+
 template <typename T>
 int dalloc(T** p, size_t n) {
   if (n == 0) n = 1;
-  if (hipMalloc((void**)p, n * sizeof(T)) != hipSuccess) return HFX_E_NOMEM;
-  return HFX_OK;
+  *p = static_cast<T*>(dev_alloc(n * sizeof(T)));
+  return *p ? HFX_OK : HFX_E_NOMEM;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  dev_free((void*)p);
+  p = nullptr;
 }
 
 }  // namespace
@@ -170,6 +264,7 @@ int grid_for(long long n, int nt, int maxblocks) {
 
 // Active (Schwarz-truncated) lists on the device, from schw/order/last.
 int upload_active(hfx_ctx* c) {
+  HIPOK(hipStreamSynchronize(c->stream));   // the lists below are replaced; no build may still read them
   for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
     const int n = c->npair[t];
     if (n == 0) continue;
@@ -187,9 +282,9 @@ int upload_active(hfx_ctx* c) {
     c->act_sorted[t] = 1;
     for (int k = 1; k < c->last[t]; ++k)
       if (asc[k] > asc[k - 1]) c->act_sorted[t] = 0;
-    if (c->d_act[t]) { hipFree(c->d_act[t]); c->d_act[t] = nullptr; }
-    if (c->d_act_schw[t]) { hipFree(c->d_act_schw[t]); c->d_act_schw[t] = nullptr; }
-    if (c->d_act_info[t]) { hipFree(c->d_act_info[t]); c->d_act_info[t] = nullptr; }
+    dfree(c->d_act[t]);
+    dfree(c->d_act_schw[t]);
+    dfree(c->d_act_info[t]);
     if (dalloc(&c->d_act[t], act.size()) || dalloc(&c->d_act_schw[t], asc.size()) ||
         dalloc(&c->d_act_info[t], info.size()))
       return HFX_E_NOMEM;
@@ -202,15 +297,15 @@ int upload_active(hfx_ctx* c) {
   // class plans and their work lists
   size_t need_masks = 0;
   for (auto& cp : c->classes) {
-    if (cp.list64) { hipFree(cp.list64); cp.list64 = nullptr; }
-    if (cp.list32) { hipFree(cp.list32); cp.list32 = nullptr; }
+    dfree(cp.list64);
+    dfree(cp.list32);
     const long long nb = c->last[cp.t1], nk = c->last[cp.t2];
     cp.cap = cp.same ? nb * (nb - 1) / 2 : nb * nk;
     if (cp.cap > 0 && dalloc(&cp.list64, (size_t)cp.cap)) return HFX_E_NOMEM;
     need_masks = std::max(need_masks, (size_t)nb * (size_t)((nk + HFX_WAVE - 1) / HFX_WAVE) * 2);
   }
   if (need_masks > c->masks_words) {
-    if (c->d_masks) { hipFree(c->d_masks); c->d_masks = nullptr; }
+    dfree(c->d_masks);
     if (dalloc(&c->d_masks, need_masks)) return HFX_E_NOMEM;
     c->masks_words = need_masks;
   }
@@ -283,8 +378,9 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   if (need > c->eblk_doubles) {
     const long long cap = std::min(kEblkDoubles, std::max(need, 2 * c->eblk_doubles));
     const long long rows = std::min(kMaxChunk, (cap / 16 + HFX_NT - 1) / HFX_NT * HFX_NT);
-    if (c->d_eblk) { HIPOK(hipFree(c->d_eblk)); c->d_eblk = nullptr; }
-    if (c->d_vmax) { HIPOK(hipFree(c->d_vmax)); c->d_vmax = nullptr; }
+    HIPOK(hipStreamSynchronize(c->stream));   // earlier chunks may still read the old buffers
+    dfree(c->d_eblk);
+    dfree(c->d_vmax);
     c->eblk_doubles = c->vmax_rows = 0;
     if (dalloc(&c->d_eblk, (size_t)cap) || dalloc(&c->d_vmax, (size_t)rows * kMaxCombo)) return HFX_E_NOMEM;
     c->eblk_doubles = cap;
@@ -371,7 +467,10 @@ int launch_maxdm(hfx_ctx* c, const double* dP) {
 
 int alloc_debug_blocks(hfx_ctx* c, ClassPlan& cp, long long cap, int prec) {
   double** b = prec ? &cp.blocks32 : &cp.blocks64;
-  if (*b) { hipFree(*b); *b = nullptr; }
+  if (*b) {
+    HIPOK(hipStreamSynchronize(c->stream));
+    dfree(*b);
+  }
   if (cap > 0 && dalloc(b, (size_t)cap * 625)) return HFX_E_NOMEM;
   if (cap > 0) {
     hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(cap * 625, 256, 4096)), dim3(256), 0, c->stream, *b,
@@ -742,16 +841,17 @@ void hfx_ctx_destroy(hfx_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
   for (auto& cp : c->classes) {
-    hipFree(cp.list64); hipFree(cp.list32); hipFree(cp.blocks64); hipFree(cp.blocks32);
+    dfree(cp.list64); dfree(cp.list32); dfree(cp.blocks64); dfree(cp.blocks32);
   }
   for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
-    hipFree(c->d_act[t]); hipFree(c->d_act_schw[t]); hipFree(c->d_act_info[t]); hipFree(c->d_pair_schw[t]); hipFree(c->d_diag_list[t]);
+    dfree(c->d_act[t]); dfree(c->d_act_schw[t]); dfree(c->d_act_info[t]); dfree(c->d_pair_schw[t]);
+    dfree(c->d_diag_list[t]);
   }
-  hipFree(c->d_pairs); hipFree(c->d_prims); hipFree(c->d_shells); hipFree(c->d_boys64); hipFree(c->d_boys32);
-  hipFree(c->d_P); hipFree(c->d_F); hipFree(c->d_X); hipFree(c->d_maxdm); hipFree(c->d_schw_out);
-  hipFree(c->d_onee_items); hipFree(c->d_sh_type); hipFree(c->d_sh_np); hipFree(c->d_sh_ao);
-  hipFree(c->d_sh_xyz); hipFree(c->d_sh_z); hipFree(c->d_sh_c); hipFree(c->d_sh_cl);
-  hipFree(c->d_counts); hipFree(c->d_stats); hipFree(c->d_eblk); hipFree(c->d_vmax); hipFree(c->d_pmax); hipFree(c->d_masks);
+  dfree(c->d_pairs); dfree(c->d_prims); dfree(c->d_shells); dfree(c->d_boys64); dfree(c->d_boys32);
+  dfree(c->d_P); dfree(c->d_F); dfree(c->d_X); dfree(c->d_maxdm); dfree(c->d_schw_out);
+  dfree(c->d_onee_items); dfree(c->d_sh_type); dfree(c->d_sh_np); dfree(c->d_sh_ao);
+  dfree(c->d_sh_xyz); dfree(c->d_sh_z); dfree(c->d_sh_c); dfree(c->d_sh_cl);
+  dfree(c->d_counts); dfree(c->d_stats); dfree(c->d_eblk); dfree(c->d_vmax); dfree(c->d_pmax); dfree(c->d_masks);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   for (int p = 0; p < 2; ++p)
@@ -897,7 +997,8 @@ int hfx_ctx_onee(hfx_ctx* c, int natom, const double* charges, const double* coo
        hipMemcpyAsync(V, d_V, nb2 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
        hipStreamSynchronize(c->stream) != hipSuccess))
     r = HFX_E_HIP;
-  hipFree(d_q); hipFree(d_xyz); hipFree(d_S); hipFree(d_T); hipFree(d_V);
+  hipStreamSynchronize(c->stream);   // nothing queued may still use the temporaries
+  dfree(d_q); dfree(d_xyz); dfree(d_S); dfree(d_T); dfree(d_V);
   return r;
 }
 
@@ -927,6 +1028,8 @@ int hfx_fock_initial_diag_device(hfx_ctx* c, const double* dP, double* dF, doubl
   if (s && s != c->stream) HIPOK(hipStreamWaitEvent(s, c->ev1, 0));
   return HFX_OK;
 }
+
+void hfx_release_cached_memory(void) { release_cached(-1); }
 
 void* hfx_ctx_stream(hfx_ctx* c) { return c ? (void*)c->stream : nullptr; }
 

@@ -82,6 +82,11 @@ class ScfResult(ctypes.Structure):
 _lib = None
 
 
+def release_cached_memory():
+    """Hand the device memory that destroyed contexts left cached back to HIP."""
+    lib().hfx_release_cached_memory()
+
+
 def lib():
     """Load libhfx.so; raises when it is missing (no CPU fallback)."""
     global _lib
@@ -115,6 +120,8 @@ def configure(L):
     L.hfx_ctx_create.argtypes = [ctypes.POINTER(MolDesc), ctypes.POINTER(vp)]
     L.hfx_ctx_destroy.argtypes = [vp]
     L.hfx_ctx_destroy.restype = None
+    L.hfx_release_cached_memory.argtypes = []
+    L.hfx_release_cached_memory.restype = None
     L.hfx_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_double]
     for n in ("hfx_fock_nd",):
         getattr(L, n).argtypes = [vp, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int]

@@ -84,6 +84,10 @@ typedef struct hfx_stats {
  * reference rejects. */
 int hfx_ctx_create(const hfx_mol_desc* desc, hfx_ctx** out);
 void hfx_ctx_destroy(hfx_ctx* ctx);
+/* Device memory released by destroyed contexts stays cached in the process
+ * for later contexts (no hipFree, which waits for the whole device, while
+ * other jobs run); this hands the cached blocks back to HIP. */
+void hfx_release_cached_memory(void);
 int hfx_set_option(hfx_ctx* ctx, int key, double value);
 
 /* F += off-diagonal two-electron part of the Fock matrix.

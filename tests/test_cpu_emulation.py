@@ -149,6 +149,24 @@ def test_device_one_electron_matches_reference(emu_fock, name):
         assert np.array_equal(dev, dev.T)
 
 
+def test_contexts_reuse_cached_device_memory(emu_fock):
+    """Contexts created after others were destroyed (blocks from the cache)
+    build the same Fock matrix; releasing the cache leaves later contexts
+    working."""
+    g = Golden("water_def2svp")
+    z = np.zeros((g.nb, g.nb))
+    ref = None
+    for k in range(3):
+        fb = emu_fock.FockBuilder(g.shells, g.pairs, nb=g.nb)
+        F = fb.non_diag(z, g.P, 1e-10)
+        fb.close()
+        if ref is None:
+            ref = F
+        assert np.array_equal(F, ref)
+        if k == 1:
+            emu_fock.release_cached_memory()
+
+
 def test_error_codes(emu_fock):
     f = emu_fock
     g = Golden("water_sto3g")
