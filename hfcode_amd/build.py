@@ -39,7 +39,8 @@ def sources():
 
 def compile_one(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    deps = [src, os.path.join(CSRC, "hfx_device.h"), os.path.join(CSRC, "hfx_kernels.h")]
+    deps = [src, os.path.join(CSRC, "hfx_device.h"), os.path.join(CSRC, "hfx_kernels.h"),
+            os.path.join(CSRC, "hfx_internal.h")]
     if os.path.basename(src).startswith("hfx_"):
         deps += [os.path.join(GEN, "eri_dispatch.h"),
                  os.path.join(ROOT, "include", "hfx.h")]

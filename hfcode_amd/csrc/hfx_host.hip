@@ -14,6 +14,7 @@
 
 #include "../../include/hfx.h"
 #include "gen/eri_dispatch.h"
+#include "hfx_internal.h"
 #include "hfx_kernels.h"
 
 using namespace hfx;
@@ -162,6 +163,13 @@ void dev_free(void* p) {
   }
   hipFree(p);
 }
+
+}  // namespace
+
+void* hfx::cached_alloc(size_t bytes) { return dev_alloc(bytes); }
+void hfx::cached_free(void* p) { dev_free(p); }
+
+namespace {
 
 template <typename T>
 int dalloc(T** p, size_t n) {

@@ -20,9 +20,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/hfx.h"
+#include "hfx_internal.h"
 
 namespace {
 
@@ -130,6 +132,30 @@ __global__ void k_scale_cols(const double* V, const double* d, double* W, int nb
   }
 }
 
+// rocBLAS handles are reused across solves (creating and destroying one
+// allocates and frees its device workspace, and hipFree waits for the whole
+// device); a handle is used by one solve at a time.
+std::mutex g_handles_mu;
+std::vector<rocblas_handle>* g_handles = new std::vector<rocblas_handle>;   // lives until the process ends
+
+rocblas_handle take_handle() {
+  {
+    std::lock_guard<std::mutex> lk(g_handles_mu);
+    if (!g_handles->empty()) {
+      rocblas_handle h = g_handles->back();
+      g_handles->pop_back();
+      return h;
+    }
+  }
+  rocblas_handle h = nullptr;
+  return rocblas_create_handle(&h) == rocblas_status_success ? h : nullptr;
+}
+
+void give_handle(rocblas_handle h) {
+  std::lock_guard<std::mutex> lk(g_handles_mu);
+  g_handles->push_back(h);
+}
+
 struct Solver {
   hfx_ctx* ctx = nullptr;
   hipStream_t st = nullptr;
@@ -147,17 +173,17 @@ struct Solver {
   hipEvent_t ev[2] = {nullptr, nullptr};
 
   ~Solver() {
-    for (double* b : bufs) hipFree(b);
-    if (info) hipFree(info);
-    if (hb) rocblas_destroy_handle(hb);
+    if (st) hipStreamSynchronize(st);   // nothing queued may still use the buffers
+    for (double* b : bufs) hfx::cached_free(b);
+    if (info) hfx::cached_free(info);
+    if (hb) give_handle(hb);
     for (hipEvent_t e : ev)
       if (e) hipEventDestroy(e);
   }
   int grid() const { return (int)std::min<long long>(2048, (n2 + 255) / 256); }
   double* alloc(long long n) {
-    double* p = nullptr;
-    if (hipMalloc(&p, (size_t)std::max<long long>(n, 1) * sizeof(double)) != hipSuccess) return nullptr;
-    bufs.push_back(p);
+    double* p = static_cast<double*>(hfx::cached_alloc((size_t)std::max<long long>(n, 1) * sizeof(double)));
+    if (p) bufs.push_back(p);
     return p;
   }
   // C = op(A) * op(B), all nb x nb
@@ -241,7 +267,7 @@ int run(Solver& s, const double* S, const double* H, int nocc, double unuc, cons
   const int nb = s.nb;
   const size_t bytes = (size_t)s.n2 * sizeof(double);
   const int ndiis = o.ndiis;
-  if (rocblas_create_handle(&s.hb) != rocblas_status_success) return HFX_E_HIP;
+  if (!(s.hb = take_handle())) return HFX_E_HIP;
   if (rocblas_set_stream(s.hb, s.st) != rocblas_status_success) return HFX_E_HIP;
   double** mats[] = {&s.S, &s.H, &s.X, &s.P, &s.Pold, &s.F, &s.Fold, &s.dP, &s.dF,
                      &s.fob, &s.T1, &s.A, &s.cmatob, &s.cmat, &s.W};
@@ -254,7 +280,7 @@ int run(Solver& s, const double* S, const double* H, int nocc, double unuc, cons
   if (!(s.eig = s.alloc(nb)) || !(s.E = s.alloc(nb)) || !(s.part = s.alloc((long long)kRedBlocks * kMaxDiis)) ||
       !(s.scal = s.alloc(kNumSlots)))
     return HFX_E_NOMEM;
-  if (hipMalloc(&s.info, sizeof(rocblas_int)) != hipSuccess) return HFX_E_NOMEM;
+  if (!(s.info = static_cast<rocblas_int*>(hfx::cached_alloc(sizeof(rocblas_int))))) return HFX_E_NOMEM;
   SCFHIP(hipEventCreate(&s.ev[0]));
   SCFHIP(hipEventCreate(&s.ev[1]));
   SCFHIP(hipMemcpyAsync(s.S, S, bytes, hipMemcpyHostToDevice, s.st));

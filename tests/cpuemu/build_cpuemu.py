@@ -25,7 +25,7 @@ def _one(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
     deps = [src, os.path.join(HERE, "hip", "hip_runtime.h"), os.path.join(HERE, "rocblas", "rocblas.h"),
             os.path.join(HERE, "rocsolver", "rocsolver.h"), os.path.join(CSRC, "hfx_device.h"),
-            os.path.join(CSRC, "hfx_kernels.h"), os.path.join(GEN, "eri_dispatch.h")]
+            os.path.join(CSRC, "hfx_kernels.h"), os.path.join(CSRC, "hfx_internal.h"), os.path.join(GEN, "eri_dispatch.h")]
     if os.path.exists(obj) and all(os.path.getmtime(obj) >= os.path.getmtime(d) for d in deps):
         return None
     r = subprocess.run(["g++"] + FLAGS + ["-c", src, "-o", obj], stdout=subprocess.PIPE,
