@@ -159,7 +159,7 @@ def _energy_check(seq, mine):
 def run_batch(a):
     from hfcode_amd import batch, molecule as M
     mols = M.batch_molecules(n=a.batch_n)
-    split = a.split if a.split > 0 else None
+    split = 1e-5 if a.split is None else (a.split if a.split > 0 else None)
     # one-at-a-time plain FP64 GPU solves of every 16th molecule: the
     # sequential baseline and each sampled job's FP64 energy
     sample = list(range(0, len(mols), max(1, len(mols) // a.seq_n)))[:a.seq_n]
@@ -209,7 +209,9 @@ def run_batch(a):
 def run_serve(a):
     from hfcode_amd import batch, molecule as M
     reqs = M.serving_requests(rate=a.rate, duration=a.duration)
-    split = a.split if a.split > 0 else None
+    # FP64 by default: with many small jobs the FP32 lists' extra launches cost more than they save
+    # (measured capacity at 40 requests/s: FP64 22.0 solves/s, split 1e-5 20.3)
+    split = a.split if a.split else None
     nseq = min(a.seq_n, len(reqs))
     log("serve: sequential FP64 solves of the first %d requests" % nseq)
     seq, seq_wall = batch.run_sequential([(m, b) for _, m, b in reqs[:nseq]])
@@ -258,7 +260,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--thr", type=float, default=1e-10)
-    ap.add_argument("--split", type=float, default=1e-5)
+    ap.add_argument("--split", type=float, default=None,
+                    help="mixed-precision split (default 1e-5 for c60 and batch, 0 = FP64 for serve)")
     ap.add_argument("--density", choices=["scf", "core"], default="scf")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
@@ -279,6 +282,8 @@ def main():
         sys.exit(2)
     if a.workload == "batch":
         return run_batch(a)
+    if a.workload == "c60" and a.split is None:
+        a.split = 1e-5
     if a.workload == "serve":
         return run_serve(a)
     import torch
