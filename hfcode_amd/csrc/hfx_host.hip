@@ -230,6 +230,7 @@ struct hfx_ctx {
   double* d_boys64 = nullptr;
   float* d_boys32 = nullptr;
   double* d_P = nullptr;
+  float* d_P32 = nullptr;         // FP32 copy of the density of a mixed-precision build
   double* d_F = nullptr;
   double* d_X = nullptr;
   double* d_maxdm = nullptr;
@@ -322,6 +323,7 @@ int upload_active(hfx_ctx* c) {
 }
 
 int ensure_mixed_lists(hfx_ctx* c) {
+  if (!c->d_P32 && dalloc(&c->d_P32, (size_t)c->nb * c->nb)) return HFX_E_NOMEM;
   for (auto& cp : c->classes)
     if (!cp.list32 && cp.cap > 0 && dalloc(&cp.list32, (size_t)cp.cap)) return HFX_E_NOMEM;
   return HFX_OK;
@@ -427,6 +429,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     g.vstride = (int)chunk;
     g.pairs = c->d_pairs;
     g.P = a.P;
+    g.P32 = c->d_P32;
     g.X = a.X;
     g.dump = dump;
     g.schw_out = a.schw_out;
@@ -446,7 +449,8 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     // two), each a contiguous range of steps
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, c->max_shm / std::max<size_t>(dig_shm, 1)));
     const int grid = rows ? grid_for(q1 - q0, HFX_NT, c->ncu * per_cu) : grid_for(q1 - q0, HFX_NT, c->ncu * 8);
-    // FP32 quartets: their blocks are stored as floats (digestion stays FP64)
+    // FP32 quartets: blocks stored as floats and contracted with the FP32 copy
+    // of P; every quartet's sums are added to X in FP64
     const void* dfn = prec == 1 ? k.digf : (jonly ? k.digj : k.dig);
     HIPOK(hipLaunchKernel(dfn, dim3(grid), dim3(HFX_NT), gargs, dig_shm, c->stream));
   }
@@ -501,6 +505,11 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode) {
   HIPOK(hipEventRecord(c->ev0, c->stream));
   if ((r = zero_build_state(c))) return r;
   if ((r = launch_maxdm(c, dP))) return r;
+  if (mixed) {   // the FP32 quartets are contracted with an FP32 copy of P
+    const size_t nb2 = (size_t)c->nb * c->nb;
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for((long long)nb2, 256, 2048)), dim3(256), 0, c->stream, dP, c->d_P32, nb2);
+    HIPOK(hipGetLastError());
+  }
   for (auto& cp : c->classes) {
     if (cp.cap <= 0) continue;
     ScreenArgs s;
@@ -856,7 +865,7 @@ void hfx_ctx_destroy(hfx_ctx* c) {
     dfree(c->d_diag_list[t]);
   }
   dfree(c->d_pairs); dfree(c->d_prims); dfree(c->d_shells); dfree(c->d_boys64); dfree(c->d_boys32);
-  dfree(c->d_P); dfree(c->d_F); dfree(c->d_X); dfree(c->d_maxdm); dfree(c->d_schw_out);
+  dfree(c->d_P); dfree(c->d_P32); dfree(c->d_F); dfree(c->d_X); dfree(c->d_maxdm); dfree(c->d_schw_out);
   dfree(c->d_onee_items); dfree(c->d_sh_type); dfree(c->d_sh_np); dfree(c->d_sh_ao);
   dfree(c->d_sh_xyz); dfree(c->d_sh_z); dfree(c->d_sh_c); dfree(c->d_sh_cl);
   dfree(c->d_counts); dfree(c->d_stats); dfree(c->d_eblk); dfree(c->d_vmax); dfree(c->d_pmax); dfree(c->d_masks);

@@ -48,6 +48,7 @@ struct DigestArgs {
   int vstride;
   const PairRec* pairs;
   const double* P;
+  const float* P32;            // P in FP32: the FP32 list's quartets are contracted in FP32
   double* X;
   double* dump;                // optional: blocks in the reference routine's layout, 625 per quartet
   double* schw_out;            // optional: sqrt(max|e|) per bra pair (initial diagonal build)
@@ -94,5 +95,6 @@ __global__ void k_maxdm(const double* P, int nb, const ShellRec* sh, int nsh, do
 __global__ void k_screen(const ScreenArgs s);
 __global__ void k_screen_diag(const DiagScreenArgs s);
 __global__ void k_add_sym(const double* X, double* F, int nb);
+__global__ void k_to_f32(const double* x, float* y, size_t n);
 
 }  // namespace hfx

@@ -258,6 +258,12 @@ __global__ void k_screen_diag(const DiagScreenArgs s) {
 }
 
 // F += X + X^T (the accumulator holds each symmetric contribution once).
+// y = (float)x: the density for the FP32 quartets' contractions
+__global__ void k_to_f32(const double* __restrict__ x, float* __restrict__ y, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+
 __global__ void k_add_sym(const double* __restrict__ X, double* __restrict__ F, int nb) {
   const long long n = (long long)nb * nb;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;

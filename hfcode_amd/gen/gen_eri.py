@@ -790,10 +790,10 @@ class Gen:
         def pload(name, ao1, n1, ao2, n2):
             for i in range(n1):
                 for j in range(n2):
-                    W("      const double %s_%d_%d = d.P[(size_t)(%s + %d) * nbf + %s + %d];" % (name, i, j, ao1, i, ao2, j))
+                    W("      const T %s_%d_%d = PA[(size_t)(%s + %d) * nbf + %s + %d];" % (name, i, j, ao1, i, ao2, j))
 
         def zero(name, n):
-            W("      " + " ".join("double %s_%d = 0;" % (name, i) for i in range(n)))
+            W("      " + " ".join("T %s_%d = 0;" % (name, i) for i in range(n)))
 
         def pin(name, n):
             W("      " + " ".join("HFX_PIN(%s_%d);" % (name, i) for i in range(n)) + " HFX_LOAD_FENCE();")
@@ -803,10 +803,10 @@ class Gen:
             LDS row `row` when the quartet's bra is the resident one"""
             W("      if (inl) {")
             for j in range(n):
-                W("        atomicAdd(&srow[%d * nbf + %s + %d], %s * %s_%d);" % (row, ao2, j, wname, name, j))
+                W("        atomicAdd(&srow[%d * nbf + %s + %d], %s * (double)%s_%d);" % (row, ao2, j, wname, name, j))
             W("      } else if (act) {")
             for j in range(n):
-                W("        atomicAdd(&X[(size_t)(%s + %d) * nbf + %s + %d], %s * %s_%d);" % (ao1, i, ao2, j, wname, name, j))
+                W("        atomicAdd(&X[(size_t)(%s + %d) * nbf + %s + %d], %s * (double)%s_%d);" % (ao1, i, ao2, j, wname, name, j))
             W("      }")
 
         W("template <bool JONLY, typename T>")
@@ -819,6 +819,11 @@ class Gen:
         W("  const bool rows = d.lds_rows != 0;")
         W("  const int lane = threadIdx.x & (HFX_WAVE - 1);")
         W("  double* X = d.X;")
+        W("  // contractions in the element type: FP64 blocks with P, FP32 blocks with its FP32 copy;")
+        W("  // each quartet's sums go to X in FP64")
+        W("  const T* PA;")
+        W("  if constexpr (sizeof(T) == sizeof(float)) PA = reinterpret_cast<const T*>(d.P32);")
+        W("  else PA = reinterpret_cast<const T*>(d.P);")
         W("  const int nsteps = (qend - d.q0 + blockDim.x - 1) / blockDim.x;")
         W("  int s0, s1;")
         W("  if (rows) {   // contiguous range of steps per workgroup")
@@ -890,13 +895,13 @@ class Gen:
         gs = 2 if ncd <= DIG_SLICE_PAIR_MAX else 1   # slices per fence (larger ones would need 2x registers)
 
         def jab_flush(ia, ib):
-            W("      jab *= wj;")
+            W("      const double jd = wj * (double)jab;")
             W("      if (rows) {")
-            W("        double r = inl ? jab : 0.0;")
+            W("        double r = inl ? jd : 0.0;")
             W("        for (int o = HFX_WAVE / 2; o > 0; o >>= 1) r += __shfl_xor(r, o);")
             W("        if (lane == 0 && r != 0.0) atomicAdd(&srow[%d * nbf + rB + %d], r);" % (ia, ib))
             W("      }")
-            W("      if (act && !inl) atomicAdd(&X[(size_t)(aoA + %d) * nbf + aoB + %d], jab);" % (ia, ib))
+            W("      if (act && !inl) atomicAdd(&X[(size_t)(aoA + %d) * nbf + aoB + %d], jd);" % (ia, ib))
 
         # Pass J, then K1 and K2 below; each pass re-derives the block pointer
         # so that element addresses are not shared, and kept live, across
@@ -912,11 +917,11 @@ class Gen:
             W("      {")
             for ib in range(nB):
                 W("      {")
-                W("      const double pab = d.P[(size_t)(aoA + %d) * nbf + aoB + %d];" % (ia, ib))
-                W("      double jab = 0;")
+                W("      const T pab = PA[(size_t)(aoA + %d) * nbf + aoB + %d];" % (ia, ib))
+                W("      T jab = 0;")
                 for ic in range(nC):
                     for id_ in range(nD):
-                        W("      { const double v = (double)bp[%d]; jab = fma(v, Pcd_%d_%d, jab); Jcd_%d = fma(v, pab, Jcd_%d); }" % (
+                        W("      { const T v = bp[%d]; jab = fma(v, Pcd_%d_%d, jab); Jcd_%d = fma(v, pab, Jcd_%d); }" % (
                             lin(ia, ib, ic, id_), ic, id_, ic * nD + id_, ic * nD + id_))
                 jab_flush(ia, ib)
                 if (ia * nB + ib + 1) % gs == 0 or (ia == nA - 1 and ib == nB - 1):
@@ -933,7 +938,7 @@ class Gen:
         W("        for (int r = 0; r < HFX_WAVE / TG; ++r) {")
         W("          if (lane / TG == r) {")
         W("            double* t = tb + (lane - r * TG) * %d;" % ncd)
-        W("            " + " ".join("t[%d] = sc * Jcd_%d;" % (k, k) for k in range(ncd)))
+        W("            " + " ".join("t[%d] = sc * (double)Jcd_%d;" % (k, k) for k in range(ncd)))
         W("          }")
         W("          wave_sync();")
         W("          for (int f0 = 0; f0 < TG * %d; f0 += HFX_WAVE) {" % ncd)
@@ -962,7 +967,7 @@ class Gen:
             for ib in range(nB):
                 for ic in range(nC):
                     for id_ in range(nD):
-                        W("      { const double v = (double)bp[%d]; Kac_%d = fma(v, Pbd_%d_%d, Kac_%d); Kad_%d = fma(v, Pbc_%d_%d, Kad_%d); }" % (
+                        W("      { const T v = bp[%d]; Kac_%d = fma(v, Pbd_%d_%d, Kac_%d); Kad_%d = fma(v, Pbc_%d_%d, Kad_%d); }" % (
                             lin(ia, ib, ic, id_), ic, ib, id_, ic, id_, ib, ic, id_))
                 if (ib + 1) % gs == 0 or ib == nB - 1:
                     pin("Kac", nC); pin("Kad", nD)
@@ -982,7 +987,7 @@ class Gen:
             for ia in range(nA):
                 for ic in range(nC):
                     for id_ in range(nD):
-                        W("      { const double v = (double)bp[%d]; Kbc_%d = fma(v, Pad_%d_%d, Kbc_%d); Kbd_%d = fma(v, Pac_%d_%d, Kbd_%d); }" % (
+                        W("      { const T v = bp[%d]; Kbc_%d = fma(v, Pad_%d_%d, Kbc_%d); Kbd_%d = fma(v, Pac_%d_%d, Kbd_%d); }" % (
                             lin(ia, ib, ic, id_), ic, ia, id_, ic, id_, ia, ic, id_))
                 if (ia + 1) % gs == 0 or ia == nA - 1:
                     pin("Kbc", nC); pin("Kbd", nD)
