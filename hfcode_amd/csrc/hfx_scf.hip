@@ -289,6 +289,12 @@ int run(Solver& s, const double* S, const double* H, int nocc, double unuc, cons
   // S^-1/2 = U diag(w^-1/2) U^T (direct_hartreefock.f90:23-29)
   SCFHIP(hipMemcpyAsync(s.T1, s.S, bytes, hipMemcpyDeviceToDevice, s.st));
   SCFOK(s.eigh(s.T1, s.eig));
+  {
+    rocblas_int info_s = 0;
+    SCFHIP(hipMemcpyAsync(&info_s, s.info, sizeof(info_s), hipMemcpyDeviceToHost, s.st));
+    SCFHIP(hipStreamSynchronize(s.st));
+    if (info_s != 0) return HFX_E_STATE;
+  }
   hipLaunchKernelGGL(k_scale_cols, dim3(s.grid()), dim3(256), 0, s.st, s.T1, s.eig, s.W, nb, 0, 0, 0.0);
   SCFOK(s.gemm(s.W, false, s.T1, true, s.X));
 
@@ -356,7 +362,10 @@ int run(Solver& s, const double* S, const double* H, int nocc, double unuc, cons
     SCFOK(s.reduce<kMaxAbsDiff>(s.P, s.Pold, nullptr, kSlotDeltap));
     SCFOK(s.dots(s.ring_r[head], s.ring_r, ndiis, kSlotDots));
     SCFHIP(hipMemcpyAsync(sc.data(), s.scal, kNumSlots * sizeof(double), hipMemcpyDeviceToHost, s.st));
+    rocblas_int info_prev = 0;   // the last eigensolve so far (S, the guess or the previous iteration)
+    SCFHIP(hipMemcpyAsync(&info_prev, s.info, sizeof(info_prev), hipMemcpyDeviceToHost, s.st));
     SCFHIP(hipStreamSynchronize(s.st));
+    if (info_prev != 0) return HFX_E_STATE;   // dsyevd did not converge
     float ms = 0.f;
     SCFHIP(hipEventElapsedTime(&ms, s.ev[0], s.ev[1]));
     fock_ms += ms;
