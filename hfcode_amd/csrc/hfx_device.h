@@ -12,6 +12,7 @@
 #define HFX_BOYS_COLS 16           // F_0 .. F_15 per grid point
 #define HFX_BOYS_STEP 16.0         // grid points per unit x
 #define HFX_BOYS_XMAX 28.0         // above: asymptotic form (boys.f90:22-24)
+#define HFX_PAIRBUF_STRIDE 25      // doubles per shell pair in the X_cd pair buffer (d x d)
 
 namespace hfx {
 

@@ -227,6 +227,7 @@ struct hfx_ctx {
   PairRec* d_pairs = nullptr;
   PrimRec* d_prims = nullptr;
   ShellRec* d_shells = nullptr;
+  double* d_pairbuf = nullptr;    // X_cd of the digestion per ket pair (HFX_PAIRBUF_STRIDE per pair)
   double* d_boys64 = nullptr;
   float* d_boys32 = nullptr;
   double* d_P = nullptr;
@@ -431,6 +432,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     g.P = a.P;
     g.P32 = c->d_P32;
     g.X = a.X;
+    g.pairbuf = c->d_pairbuf;
     g.dump = dump;
     g.schw_out = a.schw_out;
     g.thr = a.thr;
@@ -463,6 +465,9 @@ int zero_build_state(hfx_ctx* c) {
   memset(c->class_ran, 0, sizeof(c->class_ran));
   const size_t nx = (size_t)c->nb * c->nb;
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)nx, 256, 2048)), dim3(256), 0, c->stream, c->d_X, nx, 0.0);
+  const size_t npb = (size_t)c->totpairs * HFX_PAIRBUF_STRIDE;
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)npb, 256, 2048)), dim3(256), 0, c->stream, c->d_pairbuf, npb,
+                     0.0);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, c->stream, c->d_counts, 2 * HFX_NCLASSES, 0);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, c->stream, c->d_stats, 2 * HFX_NCLASSES * 3);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->d_pmax, 1);
@@ -571,6 +576,8 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode) {
       if (r) return r;
     }
   }
+  hipLaunchKernelGGL(k_pairbuf_to_x, dim3(grid_for((long long)c->totpairs * HFX_PAIRBUF_STRIDE, 256, 2048)), dim3(256), 0,
+                     c->stream, c->d_pairbuf, c->d_pairs, c->d_shells, c->totpairs, c->d_X, c->nb);
   hipLaunchKernelGGL(k_add_sym, dim3(grid_for((long long)c->nb * c->nb, 256, 2048)), dim3(256), 0, c->stream, c->d_X, dF, c->nb);
   HIPOK(hipGetLastError());
   HIPOK(hipEventRecord(c->ev1, c->stream));
@@ -625,6 +632,8 @@ int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initia
       r = launch_class(c, cls, 0, false, a, n);
     if (r) return r;
   }
+  hipLaunchKernelGGL(k_pairbuf_to_x, dim3(grid_for((long long)c->totpairs * HFX_PAIRBUF_STRIDE, 256, 2048)), dim3(256), 0,
+                     c->stream, c->d_pairbuf, c->d_pairs, c->d_shells, c->totpairs, c->d_X, c->nb);
   hipLaunchKernelGGL(k_add_sym, dim3(grid_for((long long)c->nb * c->nb, 256, 2048)), dim3(256), 0, c->stream, c->d_X, dF, c->nb);
   HIPOK(hipGetLastError());
   HIPOK(hipEventRecord(c->ev1, c->stream));
@@ -825,7 +834,8 @@ int hfx_ctx_create(const hfx_mol_desc* d, hfx_ctx** out) {
   boys_table(b64, b32);
   const size_t nb2 = (size_t)c->nb * c->nb;
   if ((r = dalloc(&c->d_pairs, c->pairs.size())) || (r = dalloc(&c->d_prims, c->prims.size())) ||
-      (r = dalloc(&c->d_shells, sr.size())) || (r = dalloc(&c->d_boys64, b64.size())) ||
+      (r = dalloc(&c->d_shells, sr.size())) ||
+      (r = dalloc(&c->d_pairbuf, (size_t)c->totpairs * HFX_PAIRBUF_STRIDE)) || (r = dalloc(&c->d_boys64, b64.size())) ||
       (r = dalloc(&c->d_boys32, b32.size())) || (r = dalloc(&c->d_P, nb2)) || (r = dalloc(&c->d_F, nb2)) ||
       (r = dalloc(&c->d_X, nb2)) || (r = dalloc(&c->d_maxdm, (size_t)c->nsh * c->nsh)) || (r = dalloc(&c->d_pmax, 1)) ||
       (r = dalloc(&c->d_schw_out, (size_t)tot)) || (r = dalloc(&c->d_counts, 2 * HFX_NCLASSES)) ||
@@ -864,7 +874,7 @@ void hfx_ctx_destroy(hfx_ctx* c) {
     dfree(c->d_act[t]); dfree(c->d_act_schw[t]); dfree(c->d_act_info[t]); dfree(c->d_pair_schw[t]);
     dfree(c->d_diag_list[t]);
   }
-  dfree(c->d_pairs); dfree(c->d_prims); dfree(c->d_shells); dfree(c->d_boys64); dfree(c->d_boys32);
+  dfree(c->d_pairs); dfree(c->d_prims); dfree(c->d_shells); dfree(c->d_pairbuf); dfree(c->d_boys64); dfree(c->d_boys32);
   dfree(c->d_P); dfree(c->d_P32); dfree(c->d_F); dfree(c->d_X); dfree(c->d_maxdm); dfree(c->d_schw_out);
   dfree(c->d_onee_items); dfree(c->d_sh_type); dfree(c->d_sh_np); dfree(c->d_sh_ao);
   dfree(c->d_sh_xyz); dfree(c->d_sh_z); dfree(c->d_sh_c); dfree(c->d_sh_cl);

@@ -50,6 +50,8 @@ struct DigestArgs {
   const double* P;
   const float* P32;            // P in FP32: the FP32 list's quartets are contracted in FP32
   double* X;
+  double* pairbuf;             // X_cd per ket pair: HFX_PAIRBUF_STRIDE doubles per global pair,
+                               // element (i, j) of the pair's shells a, b at i * n_b + j
   double* dump;                // optional: blocks in the reference routine's layout, 625 per quartet
   double* schw_out;            // optional: sqrt(max|e|) per bra pair (initial diagonal build)
   double thr;
@@ -96,5 +98,8 @@ __global__ void k_screen(const ScreenArgs s);
 __global__ void k_screen_diag(const DiagScreenArgs s);
 __global__ void k_add_sym(const double* X, double* F, int nb);
 __global__ void k_to_f32(const double* x, float* y, size_t n);
+// X += the pair buffer, block by block (every global pair once)
+__global__ void k_pairbuf_to_x(const double* pairbuf, const PairRec* pairs, const ShellRec* shells, int npairs,
+                               double* X, int nb);
 
 }  // namespace hfx

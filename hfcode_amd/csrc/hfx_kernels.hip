@@ -258,6 +258,22 @@ __global__ void k_screen_diag(const DiagScreenArgs s) {
 }
 
 // F += X + X^T (the accumulator holds each symmetric contribution once).
+// X += pair buffer: one thread per (pair, element); the pairs are distinct
+// shell pairs, so every element of X is written by at most one thread.
+__global__ void k_pairbuf_to_x(const double* __restrict__ pairbuf, const PairRec* __restrict__ pairs,
+                               const ShellRec* __restrict__ shells, int npairs, double* __restrict__ X, int nb) {
+  const long long n = (long long)npairs * HFX_PAIRBUF_STRIDE;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double v = pairbuf[i];
+    if (v == 0.0) continue;
+    const int p = (int)(i / HFX_PAIRBUF_STRIDE), e = (int)(i - (long long)p * HFX_PAIRBUF_STRIDE);
+    const PairRec& pr = pairs[p];
+    const int nbc = shells[pr.shb].ncomp;
+    const int ia = e / nbc, ib = e - ia * nbc;
+    if (ia < shells[pr.sha].ncomp) X[(size_t)(pr.aoa + ia) * nb + pr.aob + ib] += v;
+  }
+}
+
 // y = (float)x: the density for the FP32 quartets' contractions
 __global__ void k_to_f32(const double* __restrict__ x, float* __restrict__ y, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)

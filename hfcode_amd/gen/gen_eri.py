@@ -929,8 +929,9 @@ class Gen:
                 W("      }")
             W("      }")
         # X_cd through LDS: TG quartets of the wave at a time, lanes over
-        # consecutive elements, so one atomic instruction covers the row
-        # segments of a few quartets instead of 64 unrelated addresses
+        # consecutive elements, so one atomic instruction covers the
+        # contiguous pair-buffer slots of a few quartets instead of 64
+        # unrelated addresses
         W("      {")
         W("        constexpr int TG = HFX_WAVE < 16 ? HFX_WAVE : 16;")
         W("        double* tb = reinterpret_cast<double*>(hfx_smem + d.tb_offset) + (threadIdx.x / HFX_WAVE) * (TG * %d);" % ncd)
@@ -946,9 +947,12 @@ class Gen:
         W("            const bool ok = f < TG * %d;" % ncd)
         W("            const int ql = ok ? f / %d : 0, e = ok ? f - ql * %d : 0;" % (ncd, ncd))
         W("            const int src = r * TG + ql;")
-        W("            const int rc = __shfl(aoC, src) + e / %d, cd = __shfl(aoD, src) + e %% %d;" % (nD, nD))
+        # into the ket pair's slot of the pair buffer (contiguous, in the pair's own
+        # shell order), added to X once per build
+        W("            const int pq = __shfl(bk.y, src);")
+        W("            const int pe = %s;" % ("(e %% %d) * %d + e / %d" % (nD, nC, nD) if self.swap_ket else "e"))
         W("            const double v = ok ? tb[f] : 0.0;")
-        W("            if (v != 0.0) atomicAdd(&X[(size_t)rc * nbf + cd], v);")
+        W("            if (v != 0.0) atomicAdd(&d.pairbuf[(size_t)pq * HFX_PAIRBUF_STRIDE + pe], v);")
         W("          }")
         W("          wave_sync();")
         W("        }")
