@@ -64,6 +64,7 @@ struct ClassArgs {
   int vstride;             // block kernels: row length of vmax
   int lds_rows;            // fused kernels: 1 = rows of X of the resident bra pair in LDS
   int rows_offset;         // fused kernels: byte offset of those rows in the dynamic LDS
+  double* pairbuf;         // fused kernels: J_cd per ket pair (HFX_PAIRBUF_STRIDE doubles per pair)
 };
 
 // Compiler barrier between slices of an unrolled load sequence: keeps the

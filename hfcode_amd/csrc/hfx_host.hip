@@ -339,6 +339,7 @@ ClassArgs base_args(hfx_ctx* c, const double* dP, double thr) {
   a.X = c->d_X;
   a.boys64 = c->d_boys64;
   a.boys32 = c->d_boys32;
+  a.pairbuf = c->d_pairbuf;
   a.thr = thr;
   a.nb = c->nb;
   return a;

@@ -687,10 +687,13 @@ class Gen:
                 W("      }")
                 W("      if (act && !inl) atomicAdd(&X[(size_t)(aoA + %d) * nbf + aoB + %d], jab);" % (i, j))
                 W("    }")
+        # J_cd into the ket pair's contiguous pair-buffer slot (the pair's own shell order)
         W("    if (act) {")
+        W("      double* pbq = a.pairbuf + (size_t)bk.y * HFX_PAIRBUF_STRIDE;")
         for i in range(nC):
             for j in range(nD):
-                W("      atomicAdd(&X[(size_t)(aoC + %d) * nbf + aoD + %d], wj * Jcd_%d_%d);" % (i, j, i, j))
+                pe = j * nC + i if self.swap_ket else i * nD + j
+                W("      atomicAdd(&pbq[%d], wj * Jcd_%d_%d);" % (pe, i, j))
         W("    }")
 
         def krow(name, ao1, n1, row0, ao2, n2):
