@@ -67,6 +67,27 @@ struct ClassArgs {
   double* pairbuf;         // fused kernels: J_cd per ket pair (HFX_PAIRBUF_STRIDE doubles per pair)
 };
 
+// Resident bra pair for a step [lo, hi) of a bra-contiguous quartet list
+// (one quartet per thread): of the current resident bra and the bras of the
+// step's first, middle and last quartets, the one that owns the most
+// quartets of the step, the current one on ties (its rows then stay in LDS
+// without a flush).  Called by every thread of the block.
+__device__ inline int pick_resident(const int2* list, int lo, int hi, int cur) {
+  const int t = lo + (int)threadIdx.x;
+  const int mine = t < hi ? list[t].x : -1;
+  const int cand[3] = {list[lo].x, list[lo + (hi - lo - 1) / 2].x, list[hi - 1].x};
+  int best = cur, nbest = cur >= 0 ? __syncthreads_count(mine == cur) : -1;
+  for (int k = 0; k < 3; ++k) {
+    if (cand[k] == best) continue;
+    const int n = __syncthreads_count(mine == cand[k]);
+    if (n > nbest) {
+      best = cand[k];
+      nbest = n;
+    }
+  }
+  return best;
+}
+
 // Compiler barrier between slices of an unrolled load sequence: keeps the
 // scheduler from hoisting every load of a block into registers at once.
 #define HFX_LOAD_FENCE() asm volatile("" ::: "memory")

@@ -560,8 +560,8 @@ class Gen:
         """One thread per quartet; integrals, block test and J/K digestion in
         registers (light classes).  With a.lds_rows the workgroup walks a
         contiguous range of the bra-contiguous list, 256 quartets per step,
-        and keeps the rows of X of one bra pair (the bra of the step's last
-        quartet) in LDS after the Boys table: J_ab (summed over the wave
+        and keeps the rows of X of one bra pair (the one owning most of the
+        step's quartets, pick_resident) in LDS after the Boys table: J_ab (summed over the wave
         first), K_ac, K_ad, K_bc and K_bd of that bra's quartets add there and
         go to X with row-coalesced atomics when the bra changes and at the
         end; J_cd and other quartets add to X directly.  Without it (diagonal
@@ -610,7 +610,7 @@ class Gen:
         W("  };")
         W("  for (int st = s0; st < s1; st += sstride) {")
         W("    if (rows) {")
-        W("      const int tb = a.list[min((st + 1) * (int)blockDim.x, nq) - 1].x;")
+        W("      const int tb = pick_resident(a.list, st * (int)blockDim.x, min((st + 1) * (int)blockDim.x, nq), curb);")
         W("      if (tb != curb) {")
         W("        if (curb >= 0) flush();")
         W("        curb = tb;")
@@ -772,10 +772,11 @@ class Gen:
           K2: X_bc -= w/2 sum_ad e P_ad, X_bd -= w/2 sum_ac e P_ac (per b).
         Rows of A and B: with d.lds_rows the workgroup walks a contiguous
         range of the bra-contiguous list 256 quartets at a time and keeps the
-        rows of X of one bra pair (the bra of the last quartet of the step)
-        in LDS; quartets of that bra add there (X_ab summed over the wave
-        first), the rows go to X with row-coalesced atomics when the bra
-        changes and at the end.  Other quartets, and X_cd, add to X directly.
+        rows of X of one bra pair (the one owning most of the step's
+        quartets, pick_resident) in LDS; quartets of that bra add there (X_ab
+        summed over the wave first), the rows go to X with row-coalesced
+        atomics when the bra changes and at the end.  Other quartets add to X
+        directly; X_cd goes to the ket pair's slot of the pair buffer.
         A compiler barrier after each slice of loads keeps at most one slice
         of the block in registers."""
         cls = self.cls
@@ -855,7 +856,8 @@ class Gen:
         W("  for (int st = s0; st < s1; st += sstride) {")
         W("    const int it = d.q0 + st * blockDim.x + (threadIdx.x - lane);   // first quartet of the wave")
         W("    if (rows) {")
-        W("      const int tb = d.list[min(d.q0 + (st + 1) * (int)blockDim.x, qend) - 1].x;")
+        W("      const int tb = pick_resident(d.list, d.q0 + st * (int)blockDim.x,")
+        W("                                   min(d.q0 + (st + 1) * (int)blockDim.x, qend), curb);")
         W("      if (tb != curb) {")
         W("        if (curb >= 0) flush();")
         W("        curb = tb;")

@@ -50,6 +50,7 @@ static const hfx_emu_idx blockDim = {1, 1, 1};
 static const hfx_emu_idx gridDim = {1, 1, 1};
 
 inline void __syncthreads() {}
+inline int __syncthreads_count(int p) { return p ? 1 : 0; }
 #define __builtin_amdgcn_fence(order, scope) do { } while (0)
 #define __builtin_amdgcn_wave_barrier() do { } while (0)
 inline unsigned long long __ballot(bool p) { return p ? 1ull : 0ull; }
