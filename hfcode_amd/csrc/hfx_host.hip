@@ -417,9 +417,11 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     b.q0 = (int)q0;
     b.q1 = (int)q1;
     void* args[] = {&b};
-    for (int ic = 0; ic < k.ncombo; ++ic) {
+    const int ncombo = prec == 1 ? k.ncombo32 : k.ncombo;
+    for (int ic = 0; ic < ncombo; ++ic) {
       const void* fn = prec == 1 ? k.c32[ic] : k.c64[ic];
-      const size_t shm = (size_t)HFX_BOYS_ROWS * (k.combo_M[ic] + K) * (prec == 1 ? sizeof(float) : sizeof(double));
+      const int cM = prec == 1 ? k.combo_M32[ic] : k.combo_M[ic];
+      const size_t shm = (size_t)HFX_BOYS_ROWS * (cM + K) * (prec == 1 ? sizeof(float) : sizeof(double));
       HIPOK(hipLaunchKernel(fn, dim3(grid_for(q1 - q0, HFX_NT, c->ncu * 8)), dim3(HFX_NT), args, shm, c->stream));
     }
     DigestArgs g;
@@ -443,6 +445,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     g.q0 = (int)q0;
     g.q1 = (int)q1;
     g.lds_rows = rows ? 1 : 0;
+    g.ncombo = ncombo;
     g.tb_offset = (int)(dig_shm - tb_shm);
     for (int i = 0; i < 4; ++i) g.dims[i] = k.dims[i];
     g.swap_bra = k.swap_bra;

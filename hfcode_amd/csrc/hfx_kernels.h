@@ -57,6 +57,7 @@ struct DigestArgs {
   double thr;
   int nb, diag, jonly, q0, q1;
   int lds_rows;                // accumulate rows A and B of X in LDS ((nA + nB) x nb doubles)
+  int ncombo;                  // rows of vmax (combination kernels of the list's precision)
   int tb_offset;               // byte offset in LDS of the per-wave staging of X_cd (after the rows)
   int dims[4];                 // components of the oriented shells A, B, C, D
   int swap_bra, swap_ket;      // oriented A is the pair's second shell (bra / ket)

@@ -550,11 +550,37 @@ class Gen:
     def dims(self):
         return [NCOMP[x] for x in (self.A, self.B, self.C, self.D)]
 
-    def group_plan(self, split):
+    def group_plan(self, split, fp32=False):
+        """Part combinations per kernel: all of them (fused classes); one
+        (FP64 block kernels: each gets the register budget of that
+        combination alone); for the FP32 block kernels, whose values take
+        half the registers, consecutive combinations of one bra part until
+        their contracted accumulators exceed FP32_GROUP_ACC (fewer kernels,
+        so fewer passes over the primitive pairs and Boys evaluations)."""
         combos = [(bp, kp) for bp in self.bra_parts for kp in self.ket_parts]
         if not split:
             return [combos]
-        return [[c] for c in combos]
+        if not fp32:
+            return [[c] for c in combos]
+
+        def nc(ls):
+            return sum((l + 1) * (l + 2) // 2 for l in ls)
+        groups = []
+        for bp in self.bra_parts:
+            cur, cost = [], 0
+            for kp in self.ket_parts:
+                c = nc(self.erange(bp)[2]) * nc(self.frange(kp)[2])
+                if cur and cost + c > FP32_GROUP_ACC:
+                    groups.append(cur)
+                    cur, cost = [], 0
+                cur.append((bp, kp))
+                cost += c
+            if cur:
+                groups.append(cur)
+        return groups
+
+    def group_M(self, combos):
+        return max(self.combo_M(c) for c in combos)
 
     def kernel_fused(self, W, ops_prim, ops_bra, ops_ctr):
         """One thread per quartet; integrals, block test and J/K digestion in
@@ -721,18 +747,20 @@ class Gen:
         bp, kp = combo
         return max(self.erange(bp)[2]) + max(self.frange(kp)[2])
 
-    def kernel_block(self, W, ops_prim, ops_bra, ops_ctr):
-        """One kernel per L-shell part combination (so each gets the register
-        budget of that combination alone): one thread per quartet of the
-        chunk [a.q0, a.q1), its sub-block stored to a.eblk (quartet-major,
-        oriented layout); digestion follows in k_digest (heavy classes)."""
+    def kernel_block(self, W, ops_prim, ops_bra, ops_ctr, fp32=False):
+        """One kernel per group of L-shell part combinations (group_plan):
+        one thread per quartet of the chunk [a.q0, a.q1), its sub-block stored
+        to a.eblk (quartet-major, oriented layout); digestion follows in
+        dig_<class>.  FP64 kernels are eri_<class>_c<k>, FP32 ones
+        eri_<class>_f<k>."""
         cls = self.cls
         nA, nB, nC, nD = self.dims()
         n = 0
-        for gi, combos in enumerate(self.group_plan(True)):
-            M = self.combo_M(combos[0])
+        kname = "f" if fp32 else "c"
+        for gi, combos in enumerate(self.group_plan(True, fp32)):
+            M = self.group_M(combos)
             W("template <typename R>")
-            W("__global__ void __launch_bounds__(HFX_NT) eri_%s_c%d(const ClassArgs a) {" % (cls, gi))
+            W("__global__ void __launch_bounds__(HFX_NT) eri_%s_%s%d(const ClassArgs a) {" % (cls, kname, gi))
             W("  extern __shared__ char hfx_smem[];")
             W("  R* btab = reinterpret_cast<R*>(hfx_smem);")
             W("  constexpr int NB_COL = %d + BoysTerms<R>::K;" % M)
@@ -756,8 +784,7 @@ class Gen:
             n += len(el)
             W("  }")
             W("}")
-            W("template __global__ void eri_%s_c%d<double>(const ClassArgs);" % (cls, gi))
-            W("template __global__ void eri_%s_c%d<float>(const ClassArgs);" % (cls, gi))
+            W("template __global__ void eri_%s_%s%d<%s>(const ClassArgs);" % (cls, kname, gi, "float" if fp32 else "double"))
         assert n == nA * nB * nC * nD
 
     def kernel_digest(self, W, ops_ctr):
@@ -870,8 +897,7 @@ class Gen:
         W("    const int slot = iq - d.q0;")
         W("    const int2 bk = d.list[iq];")
         W("    double vmax = d.vmax[slot];")
-        for c in range(1, ncombo):
-            W("    vmax = fmax(vmax, d.vmax[(size_t)%d * d.vstride + slot]);" % c)
+        W("    for (int c = 1; c < d.ncombo; ++c) vmax = fmax(vmax, d.vmax[(size_t)c * d.vstride + slot]);")
         W("    if (d.schw_out != nullptr && valid) d.schw_out[bk.x] = sqrt(vmax);")
         W("    const bool act = valid && (d.diag || vmax >= d.thr);")
         W("    const T* blk = reinterpret_cast<const T*>(d.eblk) + (size_t)(slot >> 6) * %d + (slot & 63);" % (ne * 64))
@@ -1023,8 +1049,10 @@ class Gen:
         if split:
             W("#include \"hfx_kernels.h\"")
         W("namespace hfx {")
+        ops32 = Ops()
         if split:
             self.kernel_block(W, ops_prim, ops_bra, ops_ctr)
+            self.kernel_block(W, ops32, Ops(), Ops(), fp32=True)
             nA, nB, nC, nD = self.dims()
             # digestion in dig_<class>: 6 contractions of every element + block max
             ops_ctr.n += 13 * nA * nB * nC * nD
@@ -1038,7 +1066,9 @@ class Gen:
         info = {
             "class": self.cls, "M": self.M, "bra": self.bra_type, "ket": self.ket_type,
             "split": bool(split), "combos": ncombo,
+            "combos32": len(self.group_plan(True, True)) if split else 0,
             "flops_prim_vrr": ops_prim.n,
+            "flops_prim_vrr32": ops32.n if split else ops_prim.n,
             "flops_braprim": ops_bra.n,
             "flops_quartet": ops_ctr.n,
             "all_prims": self.all_prims,
@@ -1071,6 +1101,7 @@ class Gen:
 
 
 SPLIT_COST = 200    # classes whose fused kernel would keep more doubles live are split
+FP32_GROUP_ACC = 128   # FP32 block kernels: contracted accumulators per kernel, at most
 DIG_SLICE_PAIR_MAX = 16   # digestion: two block slices (nC x nD elements) per compiler fence when nC nD <= this
 
 
@@ -1103,10 +1134,11 @@ def main(outdir):
                 fh.write(src)
         info["flops_prim"] = info["flops_prim_vrr"] + (info["combos"] if split else 1) * boys_flops(
             info["M"], BOYS_TERMS["double"])
-        info["flops_prim_fp32"] = info["flops_prim_vrr"] + (info["combos"] if split else 1) * boys_flops(
+        info["flops_prim_fp32"] = info["flops_prim_vrr32"] + (info["combos32"] if split else 1) * boys_flops(
             info["M"], BOYS_TERMS["float"])
         info["dims"] = g.dims()
-        info["combo_M"] = [g.combo_M(grp[0]) for grp in g.group_plan(True)] if split else []
+        info["combo_M"] = [g.group_M(grp) for grp in g.group_plan(True)] if split else []
+        info["combo_M32"] = [g.group_M(grp) for grp in g.group_plan(True, True)] if split else []
         info["swap_bra"] = g.swap_bra
         info["swap_ket"] = g.swap_ket
         table[cls] = info
@@ -1117,6 +1149,8 @@ def main(outdir):
         if table[c]["split"]:
             for k in range(table[c]["combos"]):
                 D.append("template <typename R> __global__ void eri_%s_c%d(const ClassArgs);" % (c, k))
+            for k in range(table[c]["combos32"]):
+                D.append("template <typename R> __global__ void eri_%s_f%d(const ClassArgs);" % (c, k))
             D.append("template <bool JONLY, typename T> __global__ void dig_%s(const DigestArgs);" % c)
         else:
             D.append("template <typename R, bool JONLY> __global__ void eri_%s(const ClassArgs);" % c)
@@ -1124,7 +1158,8 @@ def main(outdir):
     D.append("  const void* f64; const void* f64j; const void* f32; int M; int all_prims;")
     D.append("  double flops_prim, flops_prim_fp32, flops_braprim, flops_quartet;")
     D.append("  int split; int dims[4]; int swap_bra, swap_ket;")
-    D.append("  int ncombo; int combo_M[16]; const void* c64[16]; const void* c32[16];")
+    D.append("  int ncombo; int combo_M[16]; const void* c64[16];")
+    D.append("  int ncombo32; int combo_M32[16]; const void* c32[16];   // FP32: grouped combinations")
     D.append("  const void* dig; const void* digj; const void* digf;")
     D.append("};")
     D.append("static const ClassKernels kClassKernels[%d] = {" % len(names))
@@ -1135,17 +1170,20 @@ def main(outdir):
             nc = t["combos"]
             cm = "{" + ", ".join(str(x) for x in t["combo_M"]) + "}"
             c64 = "{" + ", ".join("(const void*)&eri_%s_c%d<double>" % (c, k) for k in range(nc)) + "}"
-            c32 = "{" + ", ".join("(const void*)&eri_%s_c%d<float>" % (c, k) for k in range(nc)) + "}"
+            nc32 = t["combos32"]
+            cm32 = "{" + ", ".join(str(x) for x in t["combo_M32"]) + "}"
+            c32 = "{" + ", ".join("(const void*)&eri_%s_f%d<float>" % (c, k) for k in range(nc32)) + "}"
             dg = ("(const void*)&dig_%s<false, double>, (const void*)&dig_%s<true, double>, "
                   "(const void*)&dig_%s<false, float>" % (c, c, c))
         else:
             fns = "(const void*)&eri_%s<double, false>, (const void*)&eri_%s<double, true>, (const void*)&eri_%s<float, false>" % (c, c, c)
-            nc, cm, c64, c32 = 0, "{0}", "{nullptr}", "{nullptr}"
+            nc, cm, c64 = 0, "{0}", "{nullptr}"
+            nc32, cm32, c32 = 0, "{0}", "{nullptr}"
             dg = "nullptr, nullptr, nullptr"
-        D.append("  {%s, %d, %d, %d, %d, %d, %d, %d, {%d, %d, %d, %d}, %d, %d, %d, %s, %s, %s, %s}," % (
+        D.append("  {%s, %d, %d, %d, %d, %d, %d, %d, {%d, %d, %d, %d}, %d, %d, %d, %s, %s, %d, %s, %s, %s}," % (
             fns, t["M"], int(t["all_prims"]), t["flops_prim"], t["flops_prim_fp32"], t["flops_braprim"],
             t["flops_quartet"], int(t["split"]), *t["dims"], int(t["swap_bra"]), int(t["swap_ket"]),
-            nc, cm, c64, c32, dg))
+            nc, cm, c64, nc32, cm32, c32, dg))
     D.append("};")
     D.append("}  // namespace hfx")
     path = os.path.join(outdir, "eri_dispatch.h")

@@ -57,6 +57,33 @@ def test_class_blocks_match_oracle(emu_fock, name):
     assert len(seen) >= 20
 
 
+def test_fp32_class_blocks_match_oracle(emu_fock):
+    """Every quartet routed to FP32 (split above every bound): the FP32
+    kernels' blocks (the block classes group several L-part combinations per
+    kernel in FP32) against the oracle's FP64 blocks, to single precision."""
+    import oracle
+    g = Golden("formaldehyde_631gmix")
+    fb = emu_fock.FockBuilder(g.shells, g.pairs, nb=g.nb)
+    fb.set_mixed(1e30)
+    fb.set_debug_blocks(True)
+    fb.non_diag(np.zeros((g.nb, g.nb)), g.P, 1e-10)
+    d = g.desc()
+    seen = set()
+    for ci, cls in enumerate(emu_fock.CLASSES):
+        q, b = fb.debug_blocks(ci, prec=1)
+        for k in range(min(len(q), 6)):
+            t1, i1 = fb.pair_info(q[k, 0])
+            t2, i2 = fb.pair_info(q[k, 1])
+            s1, s2 = g.pairs[t1][0][i1 - 1]
+            s3, s4 = g.pairs[t2][0][i2 - 1]
+            dims = [NCOMP[c] for c in cls]
+            o = oracle.eri_block(d, s1, s2, s3, s4, dims, skip_negab=(cls != "slsl"))
+            mine = b[k][:int(np.prod(dims))].reshape(dims[::-1]).transpose(3, 2, 1, 0)
+            assert np.abs(mine - o).max() <= 2e-6 * np.abs(o).max() + 1e-12, (cls, s1, s2, s3, s4)
+            seen.add(cls)
+    assert len(seen) >= 20
+
+
 def test_j_only_matches_oracle(emu_fock):
     import oracle
     g = Golden("water_def2svp")
