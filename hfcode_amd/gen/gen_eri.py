@@ -551,17 +551,17 @@ class Gen:
         return [NCOMP[x] for x in (self.A, self.B, self.C, self.D)]
 
     def group_plan(self, split, fp32=False):
-        """Part combinations per kernel: all of them (fused classes); one
-        (FP64 block kernels: each gets the register budget of that
-        combination alone); for the FP32 block kernels, whose values take
-        half the registers, consecutive combinations of one bra part until
-        their contracted accumulators exceed FP32_GROUP_ACC (fewer kernels,
-        so fewer passes over the primitive pairs and Boys evaluations)."""
+        """Part combinations per kernel: all of them (fused classes); for the
+        block kernels, consecutive combinations of one bra part until their
+        contracted accumulators exceed the budget (FP64_GROUP_ACC, or
+        FP32_GROUP_ACC for FP32, whose values take half the registers): fewer
+        kernels, so fewer passes over the primitive pairs and Boys
+        evaluations, while a heavy combination keeps a kernel, and the
+        register budget, of its own."""
         combos = [(bp, kp) for bp in self.bra_parts for kp in self.ket_parts]
         if not split:
             return [combos]
-        if not fp32:
-            return [[c] for c in combos]
+        budget = FP32_GROUP_ACC if fp32 else FP64_GROUP_ACC
 
         def nc(ls):
             return sum((l + 1) * (l + 2) // 2 for l in ls)
@@ -570,7 +570,7 @@ class Gen:
             cur, cost = [], 0
             for kp in self.ket_parts:
                 c = nc(self.erange(bp)[2]) * nc(self.frange(kp)[2])
-                if cur and cost + c > FP32_GROUP_ACC:
+                if cur and cost + c > budget:
                     groups.append(cur)
                     cur, cost = [], 0
                 cur.append((bp, kp))
@@ -747,7 +747,7 @@ class Gen:
         bp, kp = combo
         return max(self.erange(bp)[2]) + max(self.frange(kp)[2])
 
-    def kernel_block(self, W, ops_prim, ops_bra, ops_ctr, fp32=False):
+    def kernel_block(self, W, ops_prim, ops_bra, ops_ctr, fp32=False, plan=None):
         """One kernel per group of L-shell part combinations (group_plan):
         one thread per quartet of the chunk [a.q0, a.q1), its sub-block stored
         to a.eblk (quartet-major, oriented layout); digestion follows in
@@ -757,7 +757,7 @@ class Gen:
         nA, nB, nC, nD = self.dims()
         n = 0
         kname = "f" if fp32 else "c"
-        for gi, combos in enumerate(self.group_plan(True, fp32)):
+        for gi, combos in enumerate(plan if plan is not None else self.group_plan(True, fp32)):
             M = self.group_M(combos)
             W("template <typename R>")
             W("__global__ void __launch_bounds__(HFX_NT) eri_%s_%s%d(const ClassArgs a) {" % (cls, kname, gi))
@@ -1049,10 +1049,13 @@ class Gen:
         if split:
             W("#include \"hfx_kernels.h\"")
         W("namespace hfx {")
-        ops32 = Ops()
         if split:
-            self.kernel_block(W, ops_prim, ops_bra, ops_ctr)
-            self.kernel_block(W, ops32, Ops(), Ops(), fp32=True)
+            # algorithmic operation counts: one kernel per part combination, whatever
+            # the grouping of the emitted kernels (so that they do not change with it)
+            singles = [[(bp, kp)] for bp in self.bra_parts for kp in self.ket_parts]
+            self.kernel_block(lambda _line: None, ops_prim, ops_bra, ops_ctr, plan=singles)
+            self.kernel_block(W, Ops(), Ops(), Ops())
+            self.kernel_block(W, Ops(), Ops(), Ops(), fp32=True)
             nA, nB, nC, nD = self.dims()
             # digestion in dig_<class>: 6 contractions of every element + block max
             ops_ctr.n += 13 * nA * nB * nC * nD
@@ -1062,13 +1065,13 @@ class Gen:
             W("template __global__ void eri_%s<double, true>(const ClassArgs);" % self.cls)
             W("template __global__ void eri_%s<float, false>(const ClassArgs);" % self.cls)
         W("}  // namespace hfx")
-        ncombo = len(self.bra_parts) * len(self.ket_parts)
         info = {
             "class": self.cls, "M": self.M, "bra": self.bra_type, "ket": self.ket_type,
-            "split": bool(split), "combos": ncombo,
+            "split": bool(split),
+            "combos": len(self.group_plan(True)) if split else 1,
             "combos32": len(self.group_plan(True, True)) if split else 0,
+            "parts": len(self.bra_parts) * len(self.ket_parts),
             "flops_prim_vrr": ops_prim.n,
-            "flops_prim_vrr32": ops32.n if split else ops_prim.n,
             "flops_braprim": ops_bra.n,
             "flops_quartet": ops_ctr.n,
             "all_prims": self.all_prims,
@@ -1101,7 +1104,8 @@ class Gen:
 
 
 SPLIT_COST = 200    # classes whose fused kernel would keep more doubles live are split
-FP32_GROUP_ACC = 128   # FP32 block kernels: contracted accumulators per kernel, at most
+FP64_GROUP_ACC = 64    # block kernels: contracted accumulators per kernel, at most (FP64)
+FP32_GROUP_ACC = 128   # the same for the FP32 block kernels
 DIG_SLICE_PAIR_MAX = 16   # digestion: two block slices (nC x nD elements) per compiler fence when nC nD <= this
 
 
@@ -1132,9 +1136,10 @@ def main(outdir):
         if old != src:
             with open(path, "w") as fh:
                 fh.write(src)
-        info["flops_prim"] = info["flops_prim_vrr"] + (info["combos"] if split else 1) * boys_flops(
+        # one Boys evaluation per part combination (split) or per quartet kernel (fused)
+        info["flops_prim"] = info["flops_prim_vrr"] + (info["parts"] if split else 1) * boys_flops(
             info["M"], BOYS_TERMS["double"])
-        info["flops_prim_fp32"] = info["flops_prim_vrr32"] + (info["combos32"] if split else 1) * boys_flops(
+        info["flops_prim_fp32"] = info["flops_prim_vrr"] + (info["parts"] if split else 1) * boys_flops(
             info["M"], BOYS_TERMS["float"])
         info["dims"] = g.dims()
         info["combo_M"] = [g.group_M(grp) for grp in g.group_plan(True)] if split else []
