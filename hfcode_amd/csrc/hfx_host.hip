@@ -13,7 +13,7 @@
 #include <vector>
 
 #include "../../include/hfx.h"
-#include "gen/eri_dispatch.h"
+#include "eri_dispatch.h"   // from the generated-kernel directory (-I)
 #include "hfx_internal.h"
 #include "hfx_kernels.h"
 

@@ -561,7 +561,8 @@ class Gen:
         combos = [(bp, kp) for bp in self.bra_parts for kp in self.ket_parts]
         if not split:
             return [combos]
-        budget = FP32_GROUP_ACC if fp32 else FP64_GROUP_ACC
+        b64, b32 = GROUP_BUDGETS.get(self.cls, (FP64_GROUP_ACC, FP32_GROUP_ACC))
+        budget = b32 if fp32 else b64
 
         def nc(ls):
             return sum((l + 1) * (l + 2) // 2 for l in ls)
@@ -1106,6 +1107,20 @@ class Gen:
 SPLIT_COST = 200    # classes whose fused kernel would keep more doubles live are split
 FP64_GROUP_ACC = 64    # block kernels: contracted accumulators per kernel, at most (FP64)
 FP32_GROUP_ACC = 128   # the same for the FP32 block kernels
+# per-class budgets picked by measurement (hfcode_amd/gen/autotune_groups.py);
+# classes not in the table use the defaults above
+GROUP_BUDGETS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "group_budgets.json")
+
+
+def load_group_budgets(path=GROUP_BUDGETS_FILE):
+    """{class: [fp64 budget, fp32 budget]} from the tuned table, {} without one."""
+    if not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        return {k: tuple(v) for k, v in json.load(fh)["budgets"].items()}
+
+
+GROUP_BUDGETS = load_group_budgets()
 DIG_SLICE_PAIR_MAX = 16   # digestion: two block slices (nC x nD elements) per compiler fence when nC nD <= this
 
 
