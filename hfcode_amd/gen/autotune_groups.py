@@ -26,7 +26,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 TUNE = os.path.join(ROOT, "hfcode_amd", "_tune")
 # variant name -> (FP64 budget, FP32 budget), contracted accumulators per kernel
-VARIANTS = {"b32": (32, 64), "b64": (64, 128), "b128": (128, 256)}
+VARIANTS = {"b16": (16, 32), "b32": (32, 64), "b64": (64, 128), "b128": (128, 256), "b256": (256, 512)}
 
 
 def variant_lib(name):
@@ -107,7 +107,7 @@ def select(measure_path, out_path=None):
         if max(t64.values()) <= 0.0:
             continue
         v64 = min(t64, key=t64.get)
-        v32 = min(t32, key=t32.get) if max(t32.values()) > 0.0 else "b64"
+        v32 = min(t32, key=t32.get) if max(t32.values()) > 0.0 else v64   # no FP32 quartets measured
         budgets[cls] = [variants[v64][0], variants[v32][1]]
         report[cls] = {"fp64_ms": t64, "fp32_ms": t32}
     out_path = out_path or gen_eri.GROUP_BUDGETS_FILE
