@@ -219,6 +219,7 @@ struct hfx_ctx {
   size_t max_shm = 0;           // LDS bytes a workgroup may use
   long long chunk_opt = 0;      // HFX_OPT_CHUNK (0: as large as the buffer allows)
   int class_ran[2][HFX_NCLASSES] = {};
+  int dig_occ[3][HFX_NCLASSES] = {};   // resident digestion workgroups per CU [fp64, fp64 J-only, fp32]
   hipEvent_t cev[2][HFX_NCLASSES][2] = {};
   int last_kind = 0;  // 0 none, 1 non-diag, 2 diag
   // device
@@ -451,13 +452,22 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     g.swap_bra = k.swap_bra;
     g.swap_ket = k.swap_ket;
     void* gargs[] = {&g};
-    // with LDS rows: as many workgroups as the LDS holds per CU (at most
-    // two), each a contiguous range of steps
-    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, c->max_shm / std::max<size_t>(dig_shm, 1)));
-    const int grid = rows ? grid_for(q1 - q0, HFX_NT, c->ncu * per_cu) : grid_for(q1 - q0, HFX_NT, c->ncu * 8);
     // FP32 quartets: blocks stored as floats and contracted with the FP32 copy
     // of P; every quartet's sums are added to X in FP64
     const void* dfn = prec == 1 ? k.digf : (jonly ? k.digj : k.dig);
+    // with LDS rows: one workgroup per resident slot (registers and LDS, as
+    // the runtime reports them; at most four per CU), each a contiguous range
+    // of steps
+    int& occ = c->dig_occ[prec == 1 ? 2 : (jonly ? 1 : 0)][cls];
+    if (rows && occ == 0) {
+      int nblk = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, dfn, HFX_NT, dig_shm) != hipSuccess || nblk < 1) {
+        (void)hipGetLastError();
+        nblk = (int)std::max<size_t>(1, std::min<size_t>(2, c->max_shm / std::max<size_t>(dig_shm, 1)));
+      }
+      occ = std::min(nblk, 4);
+    }
+    const int grid = rows ? grid_for(q1 - q0, HFX_NT, c->ncu * occ) : grid_for(q1 - q0, HFX_NT, c->ncu * 8);
     HIPOK(hipLaunchKernel(dfn, dim3(grid), dim3(HFX_NT), gargs, dig_shm, c->stream));
   }
   if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));

@@ -150,6 +150,10 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
   return hipSuccess;
 }
 #define hipLaunchKernelGGL(k, g, b, shm, st, ...) k(__VA_ARGS__)
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t) {
+  *n = 1;
+  return hipSuccess;
+}
 
 namespace hfx {
 struct ClassArgs;
