@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -225,6 +226,14 @@ struct hfx_ctx {
   // device
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // side streams for the combination kernels of a large chunk (independent
+  // writers of disjoint block elements), created on first use; fork/join events.
+  // Used only while this is the process's one live context: with several,
+  // their own streams already keep the GPU busy.
+  static constexpr int kSide = 3;
+  bool counted = false;           // holds one count of live_contexts
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork = nullptr, join[kSide] = {};
   PairRec* d_pairs = nullptr;
   PrimRec* d_prims = nullptr;
   ShellRec* d_shells = nullptr;
@@ -373,6 +382,19 @@ const long long kEblkDoubles = (1ll << 20) * 625;  // integral buffer of the blo
 const long long kMaxChunk = 1 << 21;                 // quartets per chunk, at most
 const int kMaxCombo = 16;                            // combination kernels per class, at most
 
+const long long kConcurrentChunk = 1 << 17;   // quartets of a chunk from which its combinations run side by side
+std::atomic<int> live_contexts{0};             // contexts created and not yet destroyed, process-wide
+
+int ensure_side_streams(hfx_ctx* c) {
+  if (c->fork) return HFX_OK;
+  for (int s = 0; s < hfx_ctx::kSide; ++s) {
+    HIPOK(hipStreamCreateWithFlags(&c->side[s], hipStreamNonBlocking));
+    HIPOK(hipEventCreateWithFlags(&c->join[s], hipEventDisableTiming));
+  }
+  HIPOK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+  return HFX_OK;
+}
+
 // Heavy classes: combination kernels (integrals to HBM, tiles of 64
 // quartets) + the class's digestion kernel, chunk by chunk over quartets
 // [0, n).  The digestion walks contiguous ranges of the (bra-contiguous)
@@ -419,11 +441,27 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     b.q1 = (int)q1;
     void* args[] = {&b};
     const int ncombo = prec == 1 ? k.ncombo32 : k.ncombo;
+    // a large chunk's combination kernels run side by side: each writes its own
+    // elements of the blocks and its own row of vmax, so they are independent;
+    // the digestion waits for all of them
+    const int nside = (ncombo > 1 && q1 - q0 >= kConcurrentChunk && live_contexts.load() == 1)
+                          ? std::min(ncombo - 1, (int)hfx_ctx::kSide) : 0;
+    if (nside > 0) {
+      int r = ensure_side_streams(c);
+      if (r) return r;
+      HIPOK(hipEventRecord(c->fork, c->stream));
+      for (int s = 0; s < nside; ++s) HIPOK(hipStreamWaitEvent(c->side[s], c->fork, 0));
+    }
     for (int ic = 0; ic < ncombo; ++ic) {
       const void* fn = prec == 1 ? k.c32[ic] : k.c64[ic];
       const int cM = prec == 1 ? k.combo_M32[ic] : k.combo_M[ic];
       const size_t shm = (size_t)HFX_BOYS_ROWS * (cM + K) * (prec == 1 ? sizeof(float) : sizeof(double));
-      HIPOK(hipLaunchKernel(fn, dim3(grid_for(q1 - q0, HFX_NT, c->ncu * 8)), dim3(HFX_NT), args, shm, c->stream));
+      hipStream_t st = (nside > 0 && ic % (nside + 1) != 0) ? c->side[ic % (nside + 1) - 1] : c->stream;
+      HIPOK(hipLaunchKernel(fn, dim3(grid_for(q1 - q0, HFX_NT, c->ncu * 8)), dim3(HFX_NT), args, shm, st));
+    }
+    for (int s = 0; s < nside; ++s) {
+      HIPOK(hipEventRecord(c->join[s], c->side[s]));
+      HIPOK(hipStreamWaitEvent(c->stream, c->join[s], 0));
     }
     DigestArgs g;
     memset(&g, 0, sizeof(g));
@@ -874,6 +912,8 @@ int hfx_ctx_create(const hfx_mol_desc* d, hfx_ctx** out) {
                                    hipMemcpyHostToDevice))
         return fail(HFX_E_HIP);
   }
+  c->counted = true;
+  live_contexts.fetch_add(1);
   *out = c;
   return HFX_OK;
 }
@@ -899,7 +939,13 @@ void hfx_ctx_destroy(hfx_ctx* c) {
     for (int k = 0; k < HFX_NCLASSES; ++k)
       for (int e = 0; e < 2; ++e)
         if (c->cev[p][k][e]) hipEventDestroy(c->cev[p][k][e]);
+  for (int s = 0; s < hfx_ctx::kSide; ++s) {
+    if (c->side[s]) hipStreamDestroy(c->side[s]);
+    if (c->join[s]) hipEventDestroy(c->join[s]);
+  }
+  if (c->fork) hipEventDestroy(c->fork);
   if (c->stream) hipStreamDestroy(c->stream);
+  if (c->counted) live_contexts.fetch_sub(1);
   delete c;
 }
 
