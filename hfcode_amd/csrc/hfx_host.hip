@@ -234,6 +234,7 @@ struct hfx_ctx {
   bool counted = false;           // holds one count of live_contexts
   hipStream_t side[kSide] = {};
   hipEvent_t fork = nullptr, join[kSide] = {};
+  hipEvent_t done[2] = {};        // digestion of the last chunk in each half of the integral buffer
   PairRec* d_pairs = nullptr;
   PrimRec* d_prims = nullptr;
   ShellRec* d_shells = nullptr;
@@ -391,6 +392,7 @@ int ensure_side_streams(hfx_ctx* c) {
     HIPOK(hipStreamCreateWithFlags(&c->side[s], hipStreamNonBlocking));
     HIPOK(hipEventCreateWithFlags(&c->join[s], hipEventDisableTiming));
   }
+  for (int h = 0; h < 2; ++h) HIPOK(hipEventCreateWithFlags(&c->done[h], hipEventDisableTiming));
   HIPOK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
   return HFX_OK;
 }
@@ -423,6 +425,20 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   }
   long long chunk = std::min(c->vmax_rows, (c->eblk_doubles / ne) / HFX_NT * HFX_NT);
   if (c->chunk_opt > 0) chunk = std::min(chunk, c->chunk_opt);
+  // Several chunks, one live context: the buffers split into two halves and
+  // the chunks alternate between them; the combination kernels run on the
+  // side streams, the digestion on the context's stream, so chunk j's
+  // digestion overlaps chunk j+1's integrals.  Chunk j+2 reuses chunk j's
+  // half once that digestion is done (event done[j % 2]).
+  const long long half_e = c->eblk_doubles / 2 / 64 * 64;   // doubles per half, 512-byte aligned
+  const long long half_v = c->vmax_rows / 2 * kMaxCombo;    // vmax doubles per half
+  const long long half_chunk = std::min(c->vmax_rows / 2, half_e / ne) / HFX_NT * HFX_NT;
+  const bool pipelined = n > chunk && half_chunk >= HFX_NT && live_contexts.load() == 1;
+  if (pipelined) {
+    chunk = std::min(chunk, half_chunk);
+    int r = ensure_side_streams(c);
+    if (r) return r;
+  }
   const int K = prec == 1 ? 5 : 8;
   // LDS rows of X for the digestion (off for the diagonal builds: every bra once)
   const size_t rows_shm = (size_t)(k.dims[0] + k.dims[1]) * c->nb * sizeof(double);
@@ -430,23 +446,37 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   const size_t tb_shm = (size_t)(HFX_NT / HFX_WAVE) * std::min(HFX_WAVE, 16) * k.dims[2] * k.dims[3] * sizeof(double);
   const bool rows = !a.diag && rows_shm + tb_shm <= c->max_shm;
   const size_t dig_shm = (rows ? rows_shm : 0) + tb_shm;
+  const int ncombo = prec == 1 ? k.ncombo32 : k.ncombo;
   if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
-  for (long long q0 = 0; q0 < n; q0 += chunk) {
+  if (pipelined) {
+    // the side streams start after everything already on the context's stream
+    HIPOK(hipEventRecord(c->fork, c->stream));
+    for (int s = 0; s < hfx_ctx::kSide; ++s) HIPOK(hipStreamWaitEvent(c->side[s], c->fork, 0));
+  }
+  for (long long q0 = 0, j = 0; q0 < n; q0 += chunk, ++j) {
     const long long q1 = std::min(n, q0 + chunk);
+    const int h = pipelined ? (int)(j & 1) : 0;
+    double* eblk = c->d_eblk + h * half_e;
+    double* vmax = c->d_vmax + h * half_v;
     ClassArgs b = a;
-    b.eblk = c->d_eblk;
-    b.vmax = c->d_vmax;
+    b.eblk = eblk;
+    b.vmax = vmax;
     b.vstride = (int)chunk;
     b.q0 = (int)q0;
     b.q1 = (int)q1;
     void* args[] = {&b};
-    const int ncombo = prec == 1 ? k.ncombo32 : k.ncombo;
-    // a large chunk's combination kernels run side by side: each writes its own
-    // elements of the blocks and its own row of vmax, so they are independent;
-    // the digestion waits for all of them
-    const int nside = (ncombo > 1 && q1 - q0 >= kConcurrentChunk && live_contexts.load() == 1)
-                          ? std::min(ncombo - 1, (int)hfx_ctx::kSide) : 0;
-    if (nside > 0) {
+    // A chunk's combination kernels write disjoint elements of the blocks and
+    // their own rows of vmax, so they run side by side (pipelined: all on the
+    // side streams; else, for a large chunk with one live context, on the
+    // context's stream and up to three side streams); the digestion waits for
+    // all of them.
+    int nside = 0;
+    if (pipelined) {
+      nside = std::min(ncombo, (int)hfx_ctx::kSide);
+      if (j >= 2)
+        for (int s = 0; s < nside; ++s) HIPOK(hipStreamWaitEvent(c->side[s], c->done[h], 0));
+    } else if (ncombo > 1 && q1 - q0 >= kConcurrentChunk && live_contexts.load() == 1) {
+      nside = std::min(ncombo - 1, (int)hfx_ctx::kSide);
       int r = ensure_side_streams(c);
       if (r) return r;
       HIPOK(hipEventRecord(c->fork, c->stream));
@@ -456,7 +486,8 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
       const void* fn = prec == 1 ? k.c32[ic] : k.c64[ic];
       const int cM = prec == 1 ? k.combo_M32[ic] : k.combo_M[ic];
       const size_t shm = (size_t)HFX_BOYS_ROWS * (cM + K) * (prec == 1 ? sizeof(float) : sizeof(double));
-      hipStream_t st = (nside > 0 && ic % (nside + 1) != 0) ? c->side[ic % (nside + 1) - 1] : c->stream;
+      hipStream_t st = pipelined ? c->side[ic % nside]
+                                 : (nside > 0 && ic % (nside + 1) != 0) ? c->side[ic % (nside + 1) - 1] : c->stream;
       HIPOK(hipLaunchKernel(fn, dim3(grid_for(q1 - q0, HFX_NT, c->ncu * 8)), dim3(HFX_NT), args, shm, st));
     }
     for (int s = 0; s < nside; ++s) {
@@ -467,8 +498,8 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     memset(&g, 0, sizeof(g));
     g.list = a.list;
     g.count = a.count;
-    g.eblk = c->d_eblk;
-    g.vmax = c->d_vmax;
+    g.eblk = eblk;
+    g.vmax = vmax;
     g.vstride = (int)chunk;
     g.pairs = c->d_pairs;
     g.P = a.P;
@@ -507,6 +538,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     }
     const int grid = rows ? grid_for(q1 - q0, HFX_NT, c->ncu * occ) : grid_for(q1 - q0, HFX_NT, c->ncu * 8);
     HIPOK(hipLaunchKernel(dfn, dim3(grid), dim3(HFX_NT), gargs, dig_shm, c->stream));
+    if (pipelined) HIPOK(hipEventRecord(c->done[h], c->stream));
   }
   if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));
   c->class_ran[prec][cls] = 1;
@@ -943,6 +975,8 @@ void hfx_ctx_destroy(hfx_ctx* c) {
     if (c->side[s]) hipStreamDestroy(c->side[s]);
     if (c->join[s]) hipEventDestroy(c->join[s]);
   }
+  for (int h = 0; h < 2; ++h)
+    if (c->done[h]) hipEventDestroy(c->done[h]);
   if (c->fork) hipEventDestroy(c->fork);
   if (c->stream) hipStreamDestroy(c->stream);
   if (c->counted) live_contexts.fetch_sub(1);

@@ -105,6 +105,31 @@ def test_mixed_precision_split(emu_fock):
     assert abs(dE) < 1e-8
 
 
+def test_chunked_builds(emu_fock):
+    """The block classes in chunks of 256 quartets.  With one live context the
+    chunks alternate between the two halves of the integral buffer (a chunk's
+    integrals beside the previous chunk's digestion); with a second context
+    live they run one after another.  Both give the golden Fock matrix, and
+    the chunked mixed build the unchunked one."""
+    import gc
+    gc.collect()
+    g = Golden("benzene_631gd")
+    z = np.zeros((g.nb, g.nb))
+    fb = emu_fock.FockBuilder(g.shells, g.pairs, nb=g.nb)
+    fb.set_chunk(256)
+    one = fb.non_diag(z, g.P, 1e-10)
+    assert np.abs(one - g["F_nd_1e-10"]).max() < TOL
+    other = emu_fock.FockBuilder(g.shells, g.pairs, nb=g.nb)
+    other.set_chunk(256)
+    assert np.abs(other.non_diag(z, g.P, 1e-10) - g["F_nd_1e-10"]).max() < TOL
+    other.close()
+    fb.set_mixed(1e-5)
+    mixed_chunked = fb.non_diag(z, g.P, 1e-10)
+    fb.set_chunk(0)
+    assert np.abs(mixed_chunked - fb.non_diag(z, g.P, 1e-10)).max() < 1e-12
+    fb.close()
+
+
 @pytest.mark.parametrize("name", ["water_sto3g", "water_def2svp"])
 def test_standalone_path_and_scf(emu_fock, name):
     """Own shells, one-electron matrices, Schwarz tables and SCF driver
