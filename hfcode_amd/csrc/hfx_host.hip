@@ -230,7 +230,7 @@ struct hfx_ctx {
   // writers of disjoint block elements), created on first use; fork/join events.
   // Used only while this is the process's one live context: with several,
   // their own streams already keep the GPU busy.
-  static constexpr int kSide = 3;
+  static constexpr int kSide = 2;   // with the context's stream and the null stream: four hardware queues
   bool counted = false;           // holds one count of live_contexts
   hipStream_t side[kSide] = {};
   hipEvent_t fork = nullptr, join[kSide] = {};
@@ -468,8 +468,9 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     // A chunk's combination kernels write disjoint elements of the blocks and
     // their own rows of vmax, so they run side by side (pipelined: all on the
     // side streams; else, for a large chunk with one live context, on the
-    // context's stream and up to three side streams); the digestion waits for
-    // all of them.
+    // context's stream and the side streams); the digestion waits for all of
+    // them.  Kernels go longest first (generator's cost estimate) to the
+    // stream with the least work so far.
     int nside = 0;
     if (pipelined) {
       nside = std::min(ncombo, (int)hfx_ctx::kSide);
@@ -482,12 +483,22 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
       HIPOK(hipEventRecord(c->fork, c->stream));
       for (int s = 0; s < nside; ++s) HIPOK(hipStreamWaitEvent(c->side[s], c->fork, 0));
     }
-    for (int ic = 0; ic < ncombo; ++ic) {
+    const float* cost = prec == 1 ? k.combo_cost32 : k.combo_cost;
+    int order[kMaxCombo];
+    for (int ic = 0; ic < ncombo; ++ic) order[ic] = ic;
+    std::stable_sort(order, order + ncombo, [&](int x, int y) { return cost[x] > cost[y]; });
+    const int nstream = pipelined ? nside : nside + 1;
+    double load[hfx_ctx::kSide + 1] = {};
+    for (int io = 0; io < ncombo; ++io) {
+      const int ic = order[io];
       const void* fn = prec == 1 ? k.c32[ic] : k.c64[ic];
       const int cM = prec == 1 ? k.combo_M32[ic] : k.combo_M[ic];
       const size_t shm = (size_t)HFX_BOYS_ROWS * (cM + K) * (prec == 1 ? sizeof(float) : sizeof(double));
-      hipStream_t st = pipelined ? c->side[ic % nside]
-                                 : (nside > 0 && ic % (nside + 1) != 0) ? c->side[ic % (nside + 1) - 1] : c->stream;
+      int sb = 0;
+      for (int s = 1; s < nstream; ++s)
+        if (load[s] < load[sb]) sb = s;
+      load[sb] += cost[ic];
+      hipStream_t st = pipelined ? c->side[sb] : (sb == 0 ? c->stream : c->side[sb - 1]);
       HIPOK(hipLaunchKernel(fn, dim3(grid_for(q1 - q0, HFX_NT, c->ncu * 8)), dim3(HFX_NT), args, shm, st));
     }
     for (int s = 0; s < nside; ++s) {

@@ -748,7 +748,7 @@ class Gen:
         bp, kp = combo
         return max(self.erange(bp)[2]) + max(self.frange(kp)[2])
 
-    def kernel_block(self, W, ops_prim, ops_bra, ops_ctr, fp32=False, plan=None):
+    def kernel_block(self, W, ops_prim, ops_bra, ops_ctr, fp32=False, plan=None, group_ops=None):
         """One kernel per group of L-shell part combinations (group_plan):
         one thread per quartet of the chunk [a.q0, a.q1), its sub-block stored
         to a.eblk (quartet-major, oriented layout); digestion follows in
@@ -781,6 +781,8 @@ class Gen:
                   " out[%d] = (R)o; }" % (nm, (((ia * nB + ib) * nC + ic) * nD + id_) * 64))
 
             el = self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%d" % gi, sink=store)
+            if group_ops is not None:
+                group_ops.append((ops_prim.n, ops_bra.n, ops_ctr.n))
             W("    a.vmax[(size_t)%d * a.vstride + (iq - a.q0)] = vm;" % gi)
             n += len(el)
             W("  }")
@@ -1092,6 +1094,21 @@ class Gen:
         W("}  // namespace hfx")
         return "\n".join(L) + "\n"
 
+    def group_costs(self, fp32=False):
+        """Relative cost of each emitted combination kernel of a block class:
+        its operations per primitive quartet + per bra primitive pair / 3 + per
+        quartet / 8 (about 8 primitive quartets per quartet and 3 bra
+        primitive pairs in split-valence bases).  Used to spread a chunk's
+        kernels over streams."""
+        snaps = []
+        self.kernel_block(lambda _line: None, Ops(), Ops(), Ops(), fp32=fp32, group_ops=snaps)
+        out, prev = [], (0, 0, 0)
+        for sn in snaps:
+            d = [a - b for a, b in zip(sn, prev)]
+            out.append(d[0] + d[1] / 3.0 + d[2] / 8.0)
+            prev = sn
+        return out
+
     def fused_cost(self):
         """Doubles a fused kernel keeps live: accumulators + J/K partial sums + density blocks."""
         def nc(ls):
@@ -1159,6 +1176,8 @@ def main(outdir):
         info["dims"] = g.dims()
         info["combo_M"] = [g.group_M(grp) for grp in g.group_plan(True)] if split else []
         info["combo_M32"] = [g.group_M(grp) for grp in g.group_plan(True, True)] if split else []
+        info["combo_cost"] = g.group_costs() if split else []
+        info["combo_cost32"] = g.group_costs(True) if split else []
         info["swap_bra"] = g.swap_bra
         info["swap_ket"] = g.swap_ket
         table[cls] = info
@@ -1181,6 +1200,7 @@ def main(outdir):
     D.append("  int ncombo; int combo_M[16]; const void* c64[16];")
     D.append("  int ncombo32; int combo_M32[16]; const void* c32[16];   // FP32: grouped combinations")
     D.append("  const void* dig; const void* digj; const void* digf;")
+    D.append("  float combo_cost[16]; float combo_cost32[16];   // relative cost of each combination kernel")
     D.append("};")
     D.append("static const ClassKernels kClassKernels[%d] = {" % len(names))
     for c in names:
@@ -1195,11 +1215,13 @@ def main(outdir):
             c32 = "{" + ", ".join("(const void*)&eri_%s_f%d<float>" % (c, k) for k in range(nc32)) + "}"
             dg = ("(const void*)&dig_%s<false, double>, (const void*)&dig_%s<true, double>, "
                   "(const void*)&dig_%s<false, float>" % (c, c, c))
+            dg += ", {%s}, {%s}" % (", ".join("%.1ff" % x for x in t["combo_cost"]),
+                                    ", ".join("%.1ff" % x for x in t["combo_cost32"]))
         else:
             fns = "(const void*)&eri_%s<double, false>, (const void*)&eri_%s<double, true>, (const void*)&eri_%s<float, false>" % (c, c, c)
             nc, cm, c64 = 0, "{0}", "{nullptr}"
             nc32, cm32, c32 = 0, "{0}", "{nullptr}"
-            dg = "nullptr, nullptr, nullptr"
+            dg = "nullptr, nullptr, nullptr, {0.0f}, {0.0f}"
         D.append("  {%s, %d, %d, %d, %d, %d, %d, %d, {%d, %d, %d, %d}, %d, %d, %d, %s, %s, %d, %s, %s, %s}," % (
             fns, t["M"], int(t["all_prims"]), t["flops_prim"], t["flops_prim_fp32"], t["flops_braprim"],
             t["flops_quartet"], int(t["split"]), *t["dims"], int(t["swap_bra"]), int(t["swap_ket"]),
