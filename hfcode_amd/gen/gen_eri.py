@@ -795,11 +795,13 @@ class Gen:
         quartet (tile-major blocks, so every element load is one coalesced
         wave access).  The block test (max|e| < thr skips the quartet,
         fock_build_non_diag.f90:69) uses the partial maxima the combination
-        kernels stored.  Three passes over the block, each flushing an
+        kernels stored.  Two passes over the block (three, see below), each flushing an
         accumulator as soon as it is complete:
           J:  X_ab += 2w sum_cd e P_cd (per (a,b)),  X_cd += 2w sum_ab e P_ab;
-          K1: X_ac -= w/2 sum_bd e P_bd, X_ad -= w/2 sum_bc e P_bc (per a);
-          K2: X_bc -= w/2 sum_ad e P_ad, X_bd -= w/2 sum_ac e P_ac (per b).
+          K:  X_ac -= w/2 sum_bd e P_bd, X_ad -= w/2 sum_bc e P_bc (per a),
+              X_bc -= w/2 sum_ad e P_ad, X_bd -= w/2 sum_ac e P_ac (all b, after the last a);
+        or, for the classes in KPASS_TWO, K in two passes (K1 per a: X_ac,
+        X_ad; K2 per b: X_bc, X_bd), which hold fewer registers.
         Rows of A and B: with d.lds_rows the workgroup walks a contiguous
         range of the bra-contiguous list 256 quartets at a time and keeps the
         rows of X of one bra pair (the one owning most of the step's
@@ -937,7 +939,7 @@ class Gen:
             W("      }")
             W("      if (act && !inl) atomicAdd(&X[(size_t)(aoA + %d) * nbf + aoB + %d], jd);" % (ia, ib))
 
-        # Pass J, then K1 and K2 below; each pass re-derives the block pointer
+        # Pass J, then K below; each pass re-derives the block pointer
         # so that element addresses are not shared, and kept live, across
         # passes.  (Measured on C60 and dropped: J and K1 in one pass, and all
         # of J and K in one pass with per-(a, b) partial K sums in the LDS
@@ -993,6 +995,50 @@ class Gen:
         W("      }")
         W("    }")
         W("    if constexpr (!JONLY) {")
+        # K in one pass (a outermost) or in two (K1 per a, K2 per b), per
+        # class and precision as measured on C60 (KPASS_TWO)
+        two64 = (cls, "double") in KPASS_TWO
+        two32 = (cls, "float") in KPASS_TWO
+        W("    if constexpr (sizeof(T) == sizeof(double) ? %s : %s) {" % (
+            "false" if two64 else "true", "false" if two32 else "true"))
+        # pass K: a outermost; X_ac, X_ad complete after each a, X_bc, X_bd
+        # (nB rows, kept in registers) after the last a.  Rows of P for b are
+        # reloaded per (a, b) from cache rather than held in registers.
+        W("    {")
+        W("      const T* bp = blk; HFX_PIN(bp);")
+        for ib in range(nB):
+            zero("Kbc_%d" % ib, nC)
+            zero("Kbd_%d" % ib, nD)
+        for ia in range(nA):
+            W("      {")
+            zero("Kac", nC)
+            zero("Kad", nD)
+            for id_ in range(nD):
+                W("      const T Pad_%d = PA[(size_t)(aoA + %d) * nbf + aoD + %d];" % (id_, ia, id_))
+            for ic in range(nC):
+                W("      const T Pac_%d = PA[(size_t)(aoA + %d) * nbf + aoC + %d];" % (ic, ia, ic))
+            for ib in range(nB):
+                W("      {")
+                for id_ in range(nD):
+                    W("      const T Pbd_%d = PA[(size_t)(aoB + %d) * nbf + aoD + %d];" % (id_, ib, id_))
+                for ic in range(nC):
+                    W("      const T Pbc_%d = PA[(size_t)(aoB + %d) * nbf + aoC + %d];" % (ic, ib, ic))
+                for ic in range(nC):
+                    for id_ in range(nD):
+                        W("      { const T v = bp[%d]; Kac_%d = fma(v, Pbd_%d, Kac_%d); Kad_%d = fma(v, Pbc_%d, Kad_%d);"
+                          " Kbc_%d_%d = fma(v, Pad_%d, Kbc_%d_%d); Kbd_%d_%d = fma(v, Pac_%d, Kbd_%d_%d); }" % (
+                              lin(ia, ib, ic, id_), ic, id_, ic, id_, ic, id_,
+                              ib, ic, id_, ib, ic, ib, id_, ic, ib, id_))
+                pin("Kac", nC); pin("Kad", nD); pin("Kbc_%d" % ib, nC); pin("Kbd_%d" % ib, nD)
+                W("      }")
+            flush_row("Kac", nC, "aoA", ia, ia, "aoC", "wk")
+            flush_row("Kad", nD, "aoA", ia, ia, "aoD", "wk")
+            W("      }")
+        for ib in range(nB):
+            flush_row("Kbc_%d" % ib, nC, "aoB", ib, nA + ib, "aoC", "wk")
+            flush_row("Kbd_%d" % ib, nD, "aoB", ib, nA + ib, "aoD", "wk")
+        W("    }")
+        W("    } else {")
         # pass K1
         W("    {")
         W("      const T* bp = blk; HFX_PIN(bp);")
@@ -1032,6 +1078,7 @@ class Gen:
             flush_row("Kbc", nC, "aoB", ib, nA + ib, "aoC", "wk")
             flush_row("Kbd", nD, "aoB", ib, nA + ib, "aoD", "wk")
             W("      }")
+        W("    }")
         W("    }")
         W("    }")
         W("  }")
@@ -1122,6 +1169,14 @@ class Gen:
 
 
 SPLIT_COST = 200    # classes whose fused kernel would keep more doubles live are split
+
+# Digestion classes and precisions whose K contractions keep two passes over
+# the block (K1 per a, K2 per b) instead of one: measured on the C60 /
+# 6-31G(d) build, the one-pass form took (dl|dl) FP64 from 32.6 to 38.8 ms
+# (260 registers: one wave per SIMD), (dd|ll) FP64 from 16.0 to 16.8 ms and
+# (dl|ll) FP32 from 16.3 to 17.1 ms; it is faster for the other classes
+# (e.g. (dl|ll) FP64 39.9 -> 36.9 ms, (dd|ll) FP32 4.2 -> 2.8 ms).
+KPASS_TWO = {("dldl", "double"), ("ddll", "double"), ("dlll", "float")}
 FP64_GROUP_ACC = 64    # block kernels: contracted accumulators per kernel, at most (FP64)
 FP32_GROUP_ACC = 128   # the same for the FP32 block kernels
 # per-class budgets picked by measurement (hfcode_amd/gen/autotune_groups.py);
