@@ -37,8 +37,12 @@ def _basis(name):
 def memory_estimate(st):
     """Device bytes a context for shell table `st` may take: the quartet
     lists (capacity = candidate pair-pairs of every class, two lists with
-    mixed precision off: one), the integral buffer of the block classes
-    (bounded by the library at 1.3 GB), and nb^2 matrices."""
+    mixed precision off: one), the integral buffer of the block classes and
+    its partial maxima (bounded by the library at 5.2 GB + 16 doubles per
+    quartet of a chunk while several contexts are live; a context that grew
+    its buffer while it was the only one may hold up to 5.2 GB more, which
+    the budget's distance to the device's memory covers), and nb^2
+    matrices."""
     types = {}
     for i in range(st.nsh):
         for j in range(i + 1):
@@ -49,8 +53,9 @@ def memory_estimate(st):
     for x in range(len(npairs)):
         for y in range(x, len(npairs)):
             cap += npairs[x] * npairs[y] if x != y else npairs[x] * (npairs[x] - 1) // 2
-    eblk = min(cap * 625 * 8, (1 << 18) * 625 * 8)
-    return 8 * cap + eblk + 8 * 8 * st.nb * st.nb + (1 << 27)
+    eblk = min((cap + 255) // 256 * 256 * 625, (1 << 20) * 625)      # doubles
+    vmax = min(1 << 22, (eblk // 16 + 255) // 256 * 256) * 16         # doubles
+    return 8 * cap + 8 * (eblk + vmax) + 8 * 8 * st.nb * st.nb + (1 << 27)
 
 
 DIIS_START = float("inf")   # DIIS from the second iteration on

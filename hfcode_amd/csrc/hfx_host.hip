@@ -379,8 +379,9 @@ int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   return HFX_OK;
 }
 
-const long long kEblkDoubles = (1ll << 20) * 625;  // integral buffer of the block classes, at most (5.2 GB)
-const long long kMaxChunk = 1 << 21;                 // quartets per chunk, at most
+const long long kEblkDoubles = (1ll << 20) * 625;       // integral buffer of the block classes, at most (5.2 GB) ...
+const long long kEblkDoublesAlone = (1ll << 21) * 625;  // ... or this when it grows in the one live context (10.5 GB)
+const long long kMaxChunk = 1 << 22;                      // quartets per chunk, at most
 const int kMaxCombo = 16;                            // combination kernels per class, at most
 
 const long long kConcurrentChunk = 1 << 17;   // quartets of a chunk from which its combinations run side by side
@@ -408,12 +409,15 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   if (n <= 0) return HFX_OK;
   const long long ne = (long long)k.dims[0] * k.dims[1] * k.dims[2] * k.dims[3];
   // The integral buffer grows to what this class needs for all its quartets
-  // at once (small molecules take megabytes), up to kEblkDoubles; larger
-  // lists are chunked.  Growing frees the old buffer (an implicit device
-  // synchronisation), so it grows at least twofold.
-  const long long need = std::min(kEblkDoubles, ((n + HFX_NT - 1) / HFX_NT) * HFX_NT * ne);
+  // at once (small molecules take megabytes), up to kEblkDoubles, or
+  // kEblkDoublesAlone for the process's one live context (its two halves
+  // then hold chunks as large as one buffer would with several contexts);
+  // larger lists are chunked.  Growing frees the old buffer (an implicit
+  // device synchronisation), so it grows at least twofold.
+  const long long limit = live_contexts.load() == 1 ? kEblkDoublesAlone : kEblkDoubles;
+  const long long need = std::min(limit, ((n + HFX_NT - 1) / HFX_NT) * HFX_NT * ne);
   if (need > c->eblk_doubles) {
-    const long long cap = std::min(kEblkDoubles, std::max(need, 2 * c->eblk_doubles));
+    const long long cap = std::min(limit, std::max(need, 2 * c->eblk_doubles));
     const long long rows = std::min(kMaxChunk, (cap / 16 + HFX_NT - 1) / HFX_NT * HFX_NT);
     HIPOK(hipStreamSynchronize(c->stream));   // earlier chunks may still read the old buffers
     dfree(c->d_eblk);
