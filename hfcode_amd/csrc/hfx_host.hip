@@ -387,13 +387,16 @@ const int kMaxCombo = 16;                            // combination kernels per 
 const long long kConcurrentChunk = 1 << 17;   // quartets of a chunk from which its combinations run side by side
 std::atomic<int> live_contexts{0};             // contexts created and not yet destroyed, process-wide
 
+// Creates what is still missing (a failed call leaves the handles it made,
+// which the next call keeps and hfx_ctx_destroy frees); the fork event last.
 int ensure_side_streams(hfx_ctx* c) {
   if (c->fork) return HFX_OK;
   for (int s = 0; s < hfx_ctx::kSide; ++s) {
-    HIPOK(hipStreamCreateWithFlags(&c->side[s], hipStreamNonBlocking));
-    HIPOK(hipEventCreateWithFlags(&c->join[s], hipEventDisableTiming));
+    if (!c->side[s]) HIPOK(hipStreamCreateWithFlags(&c->side[s], hipStreamNonBlocking));
+    if (!c->join[s]) HIPOK(hipEventCreateWithFlags(&c->join[s], hipEventDisableTiming));
   }
-  for (int h = 0; h < 2; ++h) HIPOK(hipEventCreateWithFlags(&c->done[h], hipEventDisableTiming));
+  for (int h = 0; h < 2; ++h)
+    if (!c->done[h]) HIPOK(hipEventCreateWithFlags(&c->done[h], hipEventDisableTiming));
   HIPOK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
   return HFX_OK;
 }
