@@ -105,6 +105,23 @@ def test_mixed_precision_split(emu_fock):
     assert abs(dE) < 1e-8
 
 
+@pytest.mark.parametrize("name", ["water_def2svp", "benzene_631gd", "formaldehyde_631gmix"])
+def test_all_fp32_build_matches_fp64(emu_fock, name):
+    """Every quartet through the FP32 kernels and the FP32 digestion (both
+    of its K forms occur among the classes): F within FP32 rounding of the
+    FP64 build (measured max |dF| 1e-6..3e-6 at max |F| ~ 15; a wrong index
+    in a contraction moves F by ~|e| |P| >= 1e-2)."""
+    g = Golden(name)
+    fb = emu_fock.FockBuilder(g.shells, g.pairs, nb=g.nb)
+    z = np.zeros((g.nb, g.nb))
+    F64 = fb.non_diag(z, g.P, 1e-10)
+    fb.set_mixed(1e30)
+    F32 = fb.non_diag(z, g.P, 1e-10)
+    q = fb.stats()["quartets"]
+    assert q[:, 0].sum() == 0 and q[:, 1].sum() > 0
+    assert np.abs(F32 - F64).max() < 2e-5
+
+
 def test_chunked_builds(emu_fock):
     """The block classes in chunks of 256 quartets.  With one live context the
     chunks alternate between the two halves of the integral buffer (a chunk's
