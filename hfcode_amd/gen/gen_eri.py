@@ -1174,7 +1174,8 @@ SPLIT_COST = 200    # classes whose fused kernel would keep more doubles live ar
 # the block (K1 per a, K2 per b) instead of one: measured on the C60 /
 # 6-31G(d) build, the one-pass form took (dl|dl) FP64 from 32.6 to 38.8 ms
 # (260 registers: one wave per SIMD), (dd|ll) FP64 from 16.0 to 16.8 ms and
-# (dl|ll) FP32 from 16.3 to 17.1 ms; it is faster for the other classes
+# (dl|ll) FP32 from 16.3 to 17.1 ms (15.7 to 16.5 ms again with the 10.5 GB
+# integral buffer of a lone context); it is faster for the other classes
 # (e.g. (dl|ll) FP64 39.9 -> 36.9 ms, (dd|ll) FP32 4.2 -> 2.8 ms).
 KPASS_TWO = {("dldl", "double"), ("ddll", "double"), ("dlll", "float")}
 FP64_GROUP_ACC = 64    # block kernels: contracted accumulators per kernel, at most (FP64)
