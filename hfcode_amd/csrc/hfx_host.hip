@@ -283,9 +283,22 @@ int grid_for(long long n, int nt, int maxblocks) {
   return (int)g;
 }
 
+// Waits for the context's stream and its side streams.  A build joins its
+// side streams into the context's stream, but one that failed between
+// launching a chunk's kernels and joining them leaves work only there.
+hipError_t sync_all_streams(hfx_ctx* c) {
+  hipError_t e = c->stream ? hipStreamSynchronize(c->stream) : hipSuccess;
+  for (int s = 0; s < hfx_ctx::kSide; ++s)
+    if (c->side[s]) {
+      const hipError_t f = hipStreamSynchronize(c->side[s]);
+      if (e == hipSuccess) e = f;
+    }
+  return e;
+}
+
 // Active (Schwarz-truncated) lists on the device, from schw/order/last.
 int upload_active(hfx_ctx* c) {
-  HIPOK(hipStreamSynchronize(c->stream));   // the lists below are replaced; no build may still read them
+  HIPOK(sync_all_streams(c));   // the lists below are replaced; no build may still read them
   for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
     const int n = c->npair[t];
     if (n == 0) continue;
@@ -422,7 +435,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   if (need > c->eblk_doubles) {
     const long long cap = std::min(limit, std::max(need, 2 * c->eblk_doubles));
     const long long rows = std::min(kMaxChunk, (cap / 16 + HFX_NT - 1) / HFX_NT * HFX_NT);
-    HIPOK(hipStreamSynchronize(c->stream));   // earlier chunks may still read the old buffers
+    HIPOK(sync_all_streams(c));   // earlier chunks may still use the old buffers
     dfree(c->d_eblk);
     dfree(c->d_vmax);
     c->eblk_doubles = c->vmax_rows = 0;
@@ -970,7 +983,7 @@ int hfx_ctx_create(const hfx_mol_desc* d, hfx_ctx** out) {
 
 void hfx_ctx_destroy(hfx_ctx* c) {
   if (!c) return;
-  if (c->stream) hipStreamSynchronize(c->stream);
+  sync_all_streams(c);
   for (auto& cp : c->classes) {
     dfree(cp.list64); dfree(cp.list32); dfree(cp.blocks64); dfree(cp.blocks32);
   }
