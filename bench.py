@@ -72,6 +72,18 @@ def _claim_stdout():
     os.dup2(2, 1)
 
 
+def _class_kernels(cls):
+    """Names of the kernels a class's FP64 span covers."""
+    path = os.path.join(ROOT, "hfcode_amd", "gen", "flops_per_class.json")
+    split = False
+    if os.path.exists(path):
+        with open(path) as fh:
+            split = json.load(fh)["classes"][cls]["split"]
+    if split:
+        return "class (%s|%s) FP64 span: eri_%s_c*<double> + dig_%s<false,double>" % (cls[:2], cls[2:], cls, cls)
+    return "class (%s|%s) FP64: eri_%s<double,false>" % (cls[:2], cls[2:], cls)
+
+
 def setup(density, thr):
     from hfcode_amd import fock, molecule as M, scf
     mol = M.c60()
@@ -117,6 +129,37 @@ def time_builds(fb, dP, dH, dF, thr, steps, warmup, torch):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return dt, class_ms / steps, class_flops, st
+
+
+def host_cpu():
+    """(threads to use, description) of the host's CPU share: OMP_NUM_THREADS
+    when the environment sets it (a GPU box's share of its host), else every
+    CPU the process may run on; the model name from /proc/cpuinfo."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else avail
+    threads = min(threads, avail)
+    model, phys = None, set()
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if ":" not in line:
+                    if "core id" in cur:
+                        phys.add((cur.get("physical id"), cur["core id"]))
+                    cur = {}
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                cur[k] = v
+                if k == "model name" and model is None:
+                    model = v
+    except OSError:
+        pass
+    return threads, dict(model=model, cpus_available=avail, physical_cores=len(phys) or None,
+                         omp_num_threads_env=env or None)
 
 
 def cpu_reference(mol, bas, P, thr, threads):
@@ -181,7 +224,9 @@ def run_batch(a):
     nbs = [r["nb"] for r in res]
     its = [r["iterations"] for r in res]
     conv = sum(r["converged"] for r in res)
-    value = len(mols) / wall
+    value = conv / wall          # converged solves only
+    unconverged = [dict(name=r["name"], nb=r["nb"], iterations=r["iterations"], last_dE=r.get("last_dE"),
+                        last_dP=r.get("last_dP")) for r in res if not r["converged"]]
     seq_rate = len(sample) / seq_wall
     dE, ndE = _energy_check(seq, [res[i] for i in sample])
     out = {
@@ -195,7 +240,8 @@ def run_batch(a):
                    "basis": "def2-SV(P)", "molecules": len(mols), "nb_min": min(nbs), "nb_max": max(nbs),
                    "nb_mean": round(float(np.mean(nbs)), 1), "workers": a.workers, "mixed_split": split,
                    "scf_driver": a.scf, "parallelism": "single GPU"},
-        "converged": conv, "iterations_mean": round(float(np.mean(its)), 2), "iterations_max": max(its),
+        "converged": conv, "unconverged": unconverged,
+        "iterations_mean": round(float(np.mean(its)), 2), "iterations_max": max(its),
         "solve_s_p50": round(float(np.median([r["setup_s"] + r["scf_s"] for r in res])), 4),
         "fock_s_fraction": round(float(sum(r["fock_s"] for r in res) / sum(r["setup_s"] + r["scf_s"] for r in res)), 3),
         "sequential_fp64_solves_per_s": round(seq_rate, 4),
@@ -265,7 +311,8 @@ def main():
     ap.add_argument("--split", type=float, default=None,
                     help="mixed-precision split (default 1e-5 for c60, 0 = FP64 for batch and serve)")
     ap.add_argument("--density", choices=["scf", "core"], default="scf")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="threads of the CPU baseline (default: OMP_NUM_THREADS, else every available CPU)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["c60", "batch", "serve"], default="c60")
     ap.add_argument("--chunk", type=int, default=0, help="quartets per block-class chunk (0: library default)")
@@ -318,9 +365,16 @@ def main():
     bmx = a.steps / dtmx
     use_mixed = err <= 1e-6
     value = bmx if use_mixed else b64
-    # roofline: dominant class kernel of the FP64 build
+    # roofline: the quartet class with the largest device time in the FP64
+    # build.  A block class is several kernels (its combination kernels and
+    # its digestion, chunk by chunk), timed as one span with HIP events on the
+    # library's stream around all of them; its algorithmic FLOPs are the
+    # frozen per-class counts (hfcode_amd/gen/flops_per_class.json) times the
+    # quartets the screening passed.
     k = int(np.argmax(cms64[:, 0]))
     ach = cfl64[k, 0] / (cms64[k, 0] * 1e-3) / 1e12
+    cname = fock.CLASSES[k]
+    kinfo = _class_kernels(cname)
     tot_fl = st64["flops_fp64"]
     q64 = int(st64["quartets"][:, 0].sum())
     pq64 = int(st64["prim_quartets"][:, 0].sum())
@@ -352,7 +406,8 @@ def main():
         "build_tflops_fp64": tot_fl * b64 / 1e12,
         "roofline": {
             "bound": "valu-fp64",
-            "kernel": "eri_%s<double>" % fock.CLASSES[k],
+            "kernel": kinfo,
+            "class": cname,
             "achieved": round(ach, 3),
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
@@ -375,7 +430,7 @@ def main():
     if os.path.exists(TRAFFIC_FILE) and a.density == "scf" and a.thr == 1e-10 and not a.chunk:
         with open(TRAFFIC_FILE) as fh:
             tr = json.load(fh)
-        if tr.get("kernel") == out["roofline"]["kernel"]:
+        if tr.get("class") == cname:
             t_s = out["roofline"]["avg_launch_ms"] * 1e-3
             out["roofline"]["traffic"] = tr["traffic_bytes"]
             out["roofline"]["traffic_source"] = tr["source"]
@@ -384,12 +439,19 @@ def main():
     out.update({k_: v for k_, v in info.items()})
     if not a.no_cpu:
         bas = M.load_basis("6-31g(d)")
-        dt, Fr = cpu_reference(mol, bas, P, a.thr, a.cpu_threads)
+        threads, cpu = host_cpu()
+        if a.cpu_threads:
+            threads = a.cpu_threads
+        dt, Fr = cpu_reference(mol, bas, P, a.thr, threads)
         if dt is not None:
-            out["cpu_baseline"] = {"value": round(1.0 / dt, 5), "unit": "Fock builds/s", "cores": a.cpu_threads,
+            out["cpu_baseline"] = {"value": round(1.0 / dt, 5), "unit": "Fock builds/s", "cores": threads,
                                    "kind": "reference",
                                    "sample": "one full C60 build (fock_build_diag + fock_build_non_diag, thr %g) "
-                                             "of the reference Fortran, flang -O3 -fopenmp, wall clock" % a.thr}
+                                             "of the reference Fortran, flang -O3 -fopenmp, OpenMP threads = the "
+                                             "process's CPU share, wall clock" % a.thr,
+                                   "cpu_model": cpu["model"], "cpus_available": cpu["cpus_available"],
+                                   "host_physical_cores": cpu["physical_cores"],
+                                   "omp_num_threads_env": cpu["omp_num_threads_env"]}
             out["gpu_fp64_vs_cpu"] = round(b64 * dt, 2)
             out["fock_maxabs_vs_reference"] = float(np.abs((Fr + H) - F64).max())
     emit(out)

@@ -104,9 +104,10 @@ def solve(mol, basis_name, maxiter=50, thr=scf.INTTHRESH, diis_start=DIIS_START,
     finally:
         fb.close()
     t3 = time.perf_counter()
+    last = res.history[-1] if res.history else (0, None, None, None)
     return dict(name=mol.title, natom=mol.natom, nb=st.nb, basis=basis_name, energy=res.energy,
                 iterations=res.iterations, converged=bool(res.converged), setup_s=t1 - t0, scf_s=t2 - t1,
-                close_s=t3 - t2, fock_s=res.t_fock, split=split)
+                close_s=t3 - t2, fock_s=res.t_fock, split=split, last_dE=last[2], last_dP=last[3])
 
 
 def _blas_one_thread():

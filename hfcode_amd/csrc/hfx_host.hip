@@ -33,6 +33,12 @@ const int kDiagClass[HFX_NPAIRTYPES] = {0, 10, 18, 25, 31, 36, 38, 40, 43, 45};
 
 const double kShthresh = 1.0e-11;  // shthresh = intthresh/10 (molprops.f90:163)
 const double kNegab = 1.0e-12;     // extrathresh = intthresh*0.01 (precalc_kabs.f90:13)
+// Primitives per shell, at most: the reference's compile-time maxprim
+// (molprops.f90:9-18).  k_screen keeps the set of its survivors' ket
+// primitive-pair counts + 1 in two 64-bit words, which needs
+// kMaxShellPrims^2 + 1 < 128.
+const int kMaxShellPrims = 10;
+static_assert(kMaxShellPrims * kMaxShellPrims + 1 < 128, "k_screen's primitive-count set holds keys below 128");
 
 int letter(int shtyp) {  // canonical pair-name letter order s < p < d < L
   switch (shtyp) {
@@ -612,11 +618,12 @@ int alloc_debug_blocks(hfx_ctx* c, ClassPlan& cp, long long cap, int prec) {
   return HFX_OK;
 }
 
-// Off-diagonal build on device arrays: F += G(P).
-int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode) {
+// Off-diagonal build on device arrays: F += G(P); quartets whose bound is
+// below `split` (> 0) in FP32.
+int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, double split) {
   if (!c->have_schwarz) return HFX_E_STATE;
   const bool jonly = mode == HFX_J_ONLY;
-  const bool mixed = c->split > 0.0 && !jonly;
+  const bool mixed = split > 0.0 && !jonly;
   if (mixed) {
     int r = ensure_mixed_lists(c);
     if (r) return r;
@@ -649,7 +656,7 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode) {
     s.stats64 = c->d_stats + 3 * cp.cls;
     s.stats32 = c->d_stats + 3 * (HFX_NCLASSES + cp.cls);
     s.thr = thr;
-    s.split = mixed ? c->split : 0.0;
+    s.split = mixed ? split : 0.0;
     s.nbra = c->last[cp.t1];
     s.nket = c->last[cp.t2];
     s.same = cp.same;
@@ -830,7 +837,7 @@ int hfx_ctx_create(const hfx_mol_desc* d, hfx_ctx** out) {
     if (letter(s.type) < 0) { delete c; return HFX_E_ARG; }
     hasP |= s.type == 2; hasD |= s.type == 3; hasL |= s.type == 12;
     s.nprim = d->nzet[i];
-    if (s.nprim <= 0 || s.nprim > d->maxprim) { delete c; return HFX_E_UNSUPPORTED; }
+    if (s.nprim <= 0 || s.nprim > d->maxprim || s.nprim > kMaxShellPrims) { delete c; return HFX_E_UNSUPPORTED; }
     s.ao = d->aosh[i] - 1;
     if (s.ao < 0 || s.ao + ncomp_of(s.type) > d->nb) { delete c; return HFX_E_ARG; }
     for (int k = 0; k < 3; ++k) s.x[k] = d->xsh[3 * i + k];
@@ -1048,7 +1055,7 @@ int hfx_fock_nd(hfx_ctx* c, const double* P, double* F, int nb, double thr, int 
   if (!c->have_schwarz) return HFX_E_STATE;
   int r;
   if ((r = copy_fock_host(c, P, F, nb))) return r;
-  if ((r = build_nd(c, c->d_P, c->d_F, thr, mode))) return r;
+  if ((r = build_nd(c, c->d_P, c->d_F, thr, mode, c->split))) return r;
   return fetch_fock_host(c, F, nb);
 }
 
@@ -1188,7 +1195,14 @@ void* hfx_ctx_stream(hfx_ctx* c) { return c ? (void*)c->stream : nullptr; }
 int hfx_ctx_nb(hfx_ctx* c) { return c ? c->nb : HFX_E_ARG; }
 
 int hfx_fock_nd_device(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, void* stream) {
-  if (!c || !dP || !dF || (mode != HFX_JK && mode != HFX_J_ONLY)) return HFX_E_ARG;
+  return c ? hfx::fock_nd_device_split(c, dP, dF, thr, mode, stream, c->split) : HFX_E_ARG;
+}
+
+}  // extern "C"
+
+int hfx::fock_nd_device_split(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, void* stream,
+                              double split) {
+  if (!c || !dP || !dF || (mode != HFX_JK && mode != HFX_J_ONLY) || !(split >= 0.0)) return HFX_E_ARG;
   hipStream_t s = (hipStream_t)stream;
   if (s && s != c->stream) {
     hipEvent_t e;
@@ -1197,11 +1211,15 @@ int hfx_fock_nd_device(hfx_ctx* c, const double* dP, double* dF, double thr, int
     HIPOK(hipStreamWaitEvent(c->stream, e, 0));
     hipEventDestroy(e);
   }
-  int r = build_nd(c, dP, dF, thr, mode);
+  int r = build_nd(c, dP, dF, thr, mode, split);
   if (r) return r;
   if (s && s != c->stream) HIPOK(hipStreamWaitEvent(s, c->ev1, 0));
   return HFX_OK;
 }
+
+double hfx::ctx_split(const hfx_ctx* c) { return c ? c->split : 0.0; }
+
+extern "C" {
 
 int hfx_fock_diag_device(hfx_ctx* c, const double* dP, double* dF, double thr, void* stream) {
   if (!c || !dP || !dF) return HFX_E_ARG;

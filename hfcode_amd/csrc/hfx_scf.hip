@@ -338,10 +338,13 @@ int run(Solver& s, const double* S, const double* H, int nocc, double unuc, cons
       if (it > 1) fullfock = false;
       nfull = 0;
     } else {         // F = F_old + G(P - P_old), non-diagonal part at thresh * fdifffac
+      // The incremental build stays FP64 whatever the context's mixed split:
+      // the split is an absolute bound, and near convergence almost every
+      // quartet of dP would fall below it.
       hipLaunchKernelGGL(k_axpy, dim3(s.grid()), dim3(256), 0, s.st, s.P, s.Pold, -1.0, s.dP, s.n2);
       SCFHIP(hipMemsetAsync(s.dF, 0, bytes, s.st));
       SCFBUILD(hfx_fock_diag_device(s.ctx, s.dP, s.dF, thresh, nullptr));
-      SCFBUILD(hfx_fock_nd_device(s.ctx, s.dP, s.dF, thresh * o.fdifffac, o.mode, nullptr));
+      SCFBUILD(hfx::fock_nd_device_split(s.ctx, s.dP, s.dF, thresh * o.fdifffac, o.mode, nullptr, 0.0));
       hipLaunchKernelGGL(k_axpy, dim3(s.grid()), dim3(256), 0, s.st, s.Fold, s.dF, 1.0, s.F, s.n2);
       if (++nfull == 15) fullfock = true;
     }

@@ -303,6 +303,7 @@ class FockBuilder:
         h = ctypes.c_void_p()
         _check(L.hfx_ctx_create(ctypes.byref(self._desc), ctypes.byref(h)))
         self._h = h
+        self.split = 0.0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -331,6 +332,7 @@ class FockBuilder:
     def set_mixed(self, split):
         """Quartets whose Schwarz x density bound is below `split` run in FP32."""
         _check(lib().hfx_set_option(self._h, OPT_MIXED_SPLIT, float(split)))
+        self.split = float(split)
 
     def set_profile(self, on=True):
         """Time every class kernel with HIP events (stats()['class_ms'])."""
@@ -351,12 +353,21 @@ class FockBuilder:
             raise ValueError("P and F must be (%d, %d)" % (nb, nb))
         return f, p
 
-    def non_diag(self, f, p, thr, mode="jk"):
+    def non_diag(self, f, p, thr, mode="jk", fp64=False):
         """F + G_offdiag(P): fock_build_non_diag (mode 'jk') or
-        fock_build_non_diag_j_only (mode 'j')."""
+        fock_build_non_diag_j_only (mode 'j').  `fp64`: every quartet in FP64
+        for this call whatever the mixed split (the SCF's incremental builds)."""
         f, p = self._prep(f, p)
         m = HFX_J_ONLY if mode == "j" else HFX_JK
-        _check(lib().hfx_fock_nd(self._h, _dp(p), _dp(f), self.nb, float(thr), m))
+        L = lib()
+        if fp64 and self.split > 0.0:
+            _check(L.hfx_set_option(self._h, OPT_MIXED_SPLIT, 0.0))
+            try:
+                _check(L.hfx_fock_nd(self._h, _dp(p), _dp(f), self.nb, float(thr), m))
+            finally:
+                _check(L.hfx_set_option(self._h, OPT_MIXED_SPLIT, self.split))
+        else:
+            _check(L.hfx_fock_nd(self._h, _dp(p), _dp(f), self.nb, float(thr), m))
         return f
 
     def diag(self, f, p, thr):

@@ -1099,18 +1099,24 @@ class Gen:
         if split:
             W("#include \"hfx_kernels.h\"")
         W("namespace hfx {")
+        ex_prim, ex_bra, ex_ctr = Ops(), Ops(), Ops()
         if split:
-            # algorithmic operation counts: one kernel per part combination, whatever
-            # the grouping of the emitted kernels (so that they do not change with it)
-            singles = [[(bp, kp)] for bp in self.bra_parts for kp in self.ket_parts]
-            self.kernel_block(lambda _line: None, ops_prim, ops_bra, ops_ctr, plan=singles)
-            self.kernel_block(W, Ops(), Ops(), Ops())
+            # algorithmic operation counts: one Boys evaluation and one vertical
+            # recursion per primitive quartet over the union of the class's
+            # targets (every part combination in one pass), whatever the
+            # grouping of the emitted kernels, which repeats the Boys
+            # evaluation and the shared recursion steps once per kernel
+            union = [[(bp, kp) for bp in self.bra_parts for kp in self.ket_parts]]
+            self.kernel_block(lambda _line: None, ops_prim, ops_bra, ops_ctr, plan=union)
+            self.kernel_block(W, ex_prim, ex_bra, ex_ctr)
             self.kernel_block(W, Ops(), Ops(), Ops(), fp32=True)
             nA, nB, nC, nD = self.dims()
             # digestion in dig_<class>: 6 contractions of every element + block max
             ops_ctr.n += 13 * nA * nB * nC * nD
+            ex_ctr.n += 13 * nA * nB * nC * nD
         else:
             self.kernel_fused(W, ops_prim, ops_bra, ops_ctr)
+            ex_prim.n, ex_bra.n, ex_ctr.n = ops_prim.n, ops_bra.n, ops_ctr.n
             W("template __global__ void eri_%s<double, false>(const ClassArgs);" % self.cls)
             W("template __global__ void eri_%s<double, true>(const ClassArgs);" % self.cls)
             W("template __global__ void eri_%s<float, false>(const ClassArgs);" % self.cls)
@@ -1124,6 +1130,9 @@ class Gen:
             "flops_prim_vrr": ops_prim.n,
             "flops_braprim": ops_bra.n,
             "flops_quartet": ops_ctr.n,
+            "exec_prim_vrr": ex_prim.n,
+            "exec_braprim": ex_bra.n,
+            "exec_quartet": ex_ctr.n,
             "all_prims": self.all_prims,
         }
         return "\n".join(L) + "\n", info
@@ -1203,8 +1212,63 @@ def boys_flops(M, K):
     return (K - 1) + (M + 1) * (2 * (K - 1) + 1) + 3
 
 
+# Algorithmic operation counts per class, frozen: the FLOPs a build is priced
+# at (bench.py's roofline) come from this committed table, not from the code
+# the generator currently emits, so kernel tuning cannot move them.
+# Written by ``python -m hfcode_amd.gen.gen_eri --freeze``; a CPU test checks
+# that the generator's union model still reproduces it.
+FLOPS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "flops_per_class.json")
+FROZEN_KEYS = ("flops_prim", "flops_prim_fp32", "flops_braprim", "flops_quartet")
+
+
+def algorithmic_counts():
+    """{class: {flops_prim, flops_prim_fp32, flops_braprim, flops_quartet, M,
+    exec_prim, exec_braprim, exec_quartet}} from the generator's model."""
+    out = {}
+    for cls in CLASSES:
+        g = Gen(cls)
+        split = g.fused_cost() > SPLIT_COST
+        _, info = g.kernel(split)
+        ex_boys = sum(boys_flops(m, BOYS_TERMS["double"]) for m in (
+            [g.group_M(grp) for grp in g.group_plan(True)] if split else [info["M"]]))
+        out[cls] = {
+            "M": info["M"], "split": bool(split),
+            "flops_prim": info["flops_prim_vrr"] + boys_flops(info["M"], BOYS_TERMS["double"]),
+            "flops_prim_fp32": info["flops_prim_vrr"] + boys_flops(info["M"], BOYS_TERMS["float"]),
+            "flops_braprim": info["flops_braprim"], "flops_quartet": info["flops_quartet"],
+            "exec_prim": info["exec_prim_vrr"] + ex_boys, "exec_braprim": info["exec_braprim"],
+            "exec_quartet": info["exec_quartet"],
+        }
+    return out
+
+
+def load_frozen(path=FLOPS_FILE):
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)["classes"]
+
+
+def freeze(path=FLOPS_FILE):
+    counts = algorithmic_counts()
+    doc = {
+        "about": "Algorithmic operations per class (add = mul = 1, fma = 2): flops_prim per primitive "
+                 "quartet (one Boys evaluation and one vertical recursion over the union of the class's "
+                 "targets, contraction into the accumulators), flops_braprim per bra primitive pair, "
+                 "flops_quartet per contracted quartet (horizontal recursion, spherical transform, six J/K "
+                 "contractions and the block maximum of every element).  exec_* count what the emitted "
+                 "kernels execute (each kernel of a block class repeats the Boys evaluation and the shared "
+                 "recursion steps); they are informational.",
+        "classes": counts,
+    }
+    with open(path, "w") as fh:
+        json.dump(doc, fh, indent=1, sort_keys=True)
+        fh.write("\n")
+
+
 def main(outdir):
     os.makedirs(outdir, exist_ok=True)
+    frozen = load_frozen()
     table = {}
     names = []
     for cls in CLASSES:
@@ -1224,11 +1288,16 @@ def main(outdir):
         if old != src:
             with open(path, "w") as fh:
                 fh.write(src)
-        # one Boys evaluation per part combination (split) or per quartet kernel (fused)
-        info["flops_prim"] = info["flops_prim_vrr"] + (info["parts"] if split else 1) * boys_flops(
-            info["M"], BOYS_TERMS["double"])
-        info["flops_prim_fp32"] = info["flops_prim_vrr"] + (info["parts"] if split else 1) * boys_flops(
-            info["M"], BOYS_TERMS["float"])
+        # algorithmic: one Boys evaluation per primitive quartet; executed: one per
+        # emitted FP64 kernel (group of part combinations) of the class
+        info["flops_prim"] = info["flops_prim_vrr"] + boys_flops(info["M"], BOYS_TERMS["double"])
+        info["flops_prim_fp32"] = info["flops_prim_vrr"] + boys_flops(info["M"], BOYS_TERMS["float"])
+        info["exec_prim"] = info["exec_prim_vrr"] + sum(
+            boys_flops(m, BOYS_TERMS["double"]) for m in ([g.group_M(grp) for grp in g.group_plan(True)]
+                                                          if split else [info["M"]]))
+        if frozen is not None:
+            for k in FROZEN_KEYS:
+                info[k] = frozen[cls][k]
         info["dims"] = g.dims()
         info["combo_M"] = [g.group_M(grp) for grp in g.group_plan(True)] if split else []
         info["combo_M32"] = [g.group_M(grp) for grp in g.group_plan(True, True)] if split else []
@@ -1295,4 +1364,7 @@ def main(outdir):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(os.path.dirname(__file__), "..", "csrc", "gen"))
+    if len(sys.argv) > 1 and sys.argv[1] == "--freeze":
+        freeze()
+    else:
+        main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(os.path.dirname(__file__), "..", "csrc", "gen"))

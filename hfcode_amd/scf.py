@@ -9,7 +9,8 @@ or anything with the same four methods.  Steps, as the reference takes them:
 * iteration i (:98-223): full build (initial_diag + non_diag at i = 1,
   diag + non_diag when a full rebuild is due) or the incremental build on
   dP = P - P_old with the non-diagonal threshold x0.1, full rebuild forced
-  every 15 incremental steps (:100-121);
+  every 15 incremental steps (:100-121); with a mixed-precision builder the
+  incremental builds still run in FP64;
 * E_elec = 1/2 sum P (H + F) (:123-131); DIIS error S^-1/2 (FPS - SPF) S^-1/2
   (:133-144); DIIS over up to 8 Fock matrices with 1.05 damping of all but
   the lowest-energy entry and a 1.5 coefficient cap (:161-199);
@@ -104,7 +105,12 @@ def rhf(builder, S, H, nocc, unuc, maxiter=MAXITER, ndiis=NDIIS, conv_p=CONV_P, 
             delp = p - pold
             z = np.zeros((nb, nb))
             delf = builder.diag(z, delp, thresh)
-            delf = builder.non_diag(delf, delp, thresh * FDIFFFAC)
+            # FP64 whatever the builder's mixed split (an absolute bound: near
+            # convergence almost every quartet of dP would fall below it)
+            if getattr(builder, "split", 0.0):
+                delf = builder.non_diag(delf, delp, thresh * FDIFFFAC, fp64=True)
+            else:
+                delf = builder.non_diag(delf, delp, thresh * FDIFFFAC)
             f = faobold + delf
             nfull += 1
             if nfull == 15:

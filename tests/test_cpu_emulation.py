@@ -255,3 +255,24 @@ def test_error_codes(emu_fock):
         fb.non_diag(np.zeros((g.nb, g.nb)), g.P, 1e-10)
     with pytest.raises(ValueError):
         fb.diag(np.zeros((3, 3)), g.P, 1e-10)
+
+
+def test_more_than_ten_primitives_rejected(emu_fock):
+    """A shell with more primitives than the reference's maxprim (10,
+    molprops.f90:9-18) is refused at context creation: the screening keeps
+    the set of survivors' primitive-pair counts + 1 in 128 bits."""
+    g = Golden("water_sto3g")
+    sh = {k: np.array(v) for k, v in g.shells.items()}
+    mp = 16
+    for k in ("zeta", "djk", "djkl"):
+        a = np.zeros((len(sh["shtyp"]), mp))
+        a[:, :sh[k].shape[1]] = sh[k]
+        sh[k] = a
+    ok = emu_fock.FockBuilder(sh, nb=g.nb)       # a leading dimension of 16 is fine
+    ok.close()
+    sh["nzet"] = sh["nzet"].copy()
+    sh["nzet"][0] = 11
+    sh["zeta"][0, :11] = np.geomspace(100.0, 0.1, 11)
+    sh["djk"][0, :11] = 0.1
+    with pytest.raises(emu_fock.HfxError, match="unsupported"):
+        emu_fock.FockBuilder(sh, nb=g.nb)
