@@ -199,28 +199,42 @@ def _energy_check(seq, mine):
     return (max(d) if d else None), len(d)
 
 
+def _sequential(jobs):
+    """One-at-a-time plain FP64 GPU solves: the per-job path (host SCF driver
+    around a single-molecule context) and a batch context of one (device
+    driver); the faster of the two is the baseline."""
+    from hfcode_amd import batch
+    seq_host, wall_host = batch.run_sequential(jobs)
+    t0 = time.perf_counter()
+    seq_dev = [batch.solve_group([m], b)[0] for m, b in jobs]
+    wall_dev = time.perf_counter() - t0
+    if wall_dev < wall_host:
+        return seq_dev, wall_dev, "batch context of one (device SCF driver, graphs)", dict(per_job_host=wall_host)
+    return seq_host, wall_host, "single-molecule context, host SCF driver", dict(batch_of_one=wall_dev)
+
+
 def run_batch(a):
     from hfcode_amd import batch, molecule as M
     mols = M.batch_molecules(n=a.batch_n)
-    # FP64 by default (measured: 5.90 solves/s against 5.48 with split 1e-5; these molecules are too
-    # small for the FP32 lists to pay for their extra launches)
-    split = a.split if a.split else None
-    # one-at-a-time plain FP64 GPU solves of every 16th molecule: the
-    # sequential baseline and each sampled job's FP64 energy
+    split = a.split if a.split is not None else 1e-5      # the configuration names mixed precision
     sample = list(range(0, len(mols), max(1, len(mols) // a.seq_n)))[:a.seq_n]
     log("batch: sequential FP64 solves of %d sampled molecules" % len(sample))
-    seq, seq_wall = batch.run_sequential([(mols[i], "def2-sv(p)") for i in sample])
-    log("batch: sequential %.1f s" % seq_wall)
+    seq, seq_wall, seq_kind, seq_other = _sequential([(mols[i], "def2-sv(p)") for i in sample])
+    log("batch: sequential %.1f s (%s)" % (seq_wall, seq_kind))
     peak = [0]
 
     def progress(done, total, el):
         u = _device_used_bytes()
         if u:
             peak[0] = max(peak[0], u)
-        if done % 16 == 0 or done == total:
+        if done % 64 == 0 or done == total:
             log("batch: %d/%d solved, %.1f s" % (done, total, el))
-    res, wall = batch.run_batch(mols, "def2-sv(p)", workers=a.workers, progress=progress, split=split,
-                                device_scf=a.scf == "device")
+    if a.path == "batched":
+        res, wall = batch.run_batched(mols, "def2-sv(p)", group=a.group, workers=a.workers, split=split or None,
+                                      progress=progress)
+    else:
+        res, wall = batch.run_batch(mols, "def2-sv(p)", workers=a.workers, progress=progress, split=split or None,
+                                    device_scf=a.scf == "device")
     nbs = [r["nb"] for r in res]
     its = [r["iterations"] for r in res]
     conv = sum(r["converged"] for r in res)
@@ -238,16 +252,16 @@ def run_batch(a):
                 "ideal bonds + 0.05 A Gaussian jitter, seed 2024",
         "config": {"workload": "batched RHF solves (core guess, DIIS from iteration 2, thr 1e-10, <= 50 iterations)",
                    "basis": "def2-SV(P)", "molecules": len(mols), "nb_min": min(nbs), "nb_max": max(nbs),
-                   "nb_mean": round(float(np.mean(nbs)), 1), "workers": a.workers, "mixed_split": split,
-                   "scf_driver": a.scf, "parallelism": "single GPU"},
+                   "nb_mean": round(float(np.mean(nbs)), 1), "path": a.path, "group": a.group,
+                   "workers": a.workers, "mixed_split": split, "parallelism": "single GPU"},
         "converged": conv, "unconverged": unconverged,
         "iterations_mean": round(float(np.mean(its)), 2), "iterations_max": max(its),
-        "solve_s_p50": round(float(np.median([r["setup_s"] + r["scf_s"] for r in res])), 4),
-        "fock_s_fraction": round(float(sum(r["fock_s"] for r in res) / sum(r["setup_s"] + r["scf_s"] for r in res)), 3),
         "sequential_fp64_solves_per_s": round(seq_rate, 4),
+        "sequential_kind": seq_kind, "sequential_other_wall_s": seq_other,
         "sequential_sample": "every %dth molecule (%d), one at a time, FP64" % (sample[1] - sample[0] if len(sample) > 1 else 1, len(sample)),
         "vs_sequential": round(value / seq_rate, 3),
         "energy_error_max_Eh": dE, "energy_error_jobs": ndE,
+        "energy_error_note": "batch result (mixed precision) minus the same molecule's sequential FP64 solve",
         "peak_device_bytes": peak[0] or None,
         "wall_s": round(wall, 3),
     }
@@ -257,25 +271,26 @@ def run_batch(a):
 def run_serve(a):
     from hfcode_amd import batch, molecule as M
     reqs = M.serving_requests(rate=a.rate, duration=a.duration)
-    # FP64 by default: with many small jobs the FP32 lists' extra launches cost more than they save
-    # (measured capacity at 40 requests/s: FP64 22.0 solves/s, split 1e-5 20.3)
     split = a.split if a.split else None
     nseq = min(a.seq_n, len(reqs))
     log("serve: sequential FP64 solves of the first %d requests" % nseq)
-    seq, seq_wall = batch.run_sequential([(m, b) for _, m, b in reqs[:nseq]])
-    log("serve: sequential %.1f s" % seq_wall)
-    srv = batch.Server(workers=a.workers, split=split, device_scf=a.scf == "device")
+    seq, seq_wall, seq_kind, seq_other = _sequential([(m, b) for _, m, b in reqs[:nseq]])
+    log("serve: sequential %.1f s (%s)" % (seq_wall, seq_kind))
+    if a.path == "batched":
+        srv = batch.BatchServer(workers=a.workers, group=a.group, split=split)
+    else:
+        srv = batch.Server(workers=a.workers, split=split, device_scf=a.scf == "device")
     peak = [0]
 
     def progress(done, total, el):
         u = _device_used_bytes()
         if u:
             peak[0] = max(peak[0], u)
-        if done % 50 == 0 or done == total:
+        if done % 100 == 0 or done == total:
             log("serve: %d/%d requests done, %.1f s" % (done, total, el))
     res, wall = srv.run(reqs, progress=progress)
     lat = batch.latency_summary(res)
-    ok = [r for r in res if not r.get("error")]
+    ok = [r for r in res if not r.get("error") and r.get("converged")]
     rate = len(ok) / wall
     seq_rate = nseq / seq_wall
     dE, ndE = _energy_check(seq, res[:nseq])
@@ -286,12 +301,13 @@ def run_serve(a):
         "dtype": "f64+f32" if split else "f64",
         "data": "synthetic: Poisson arrivals, seed 7; chain molecules 5-120 atoms log-uniform",
         "config": {"workload": "serving loop, full RHF per request (atomic-density guess, thr 1e-10)",
-                   "rate_per_s": a.rate, "duration_s": a.duration, "workers": a.workers, "mixed_split": split,
-                   "scf_driver": a.scf, "parallelism": "single GPU"},
+                   "rate_per_s": a.rate, "duration_s": a.duration, "path": a.path, "group": a.group,
+                   "workers": a.workers, "mixed_split": split, "parallelism": "single GPU"},
         "requests": len(reqs), "completed": len(ok), "converged": sum(r.get("converged", False) for r in res),
         "p50_s": round(lat["p50_s"], 4), "p99_s": round(lat["p99_s"], 4), "mean_s": round(lat["mean_s"], 4),
         "max_s": round(lat["max_s"], 4), "solves_per_s": round(rate, 3),
         "sequential_fp64_solves_per_s": round(seq_rate, 4),
+        "sequential_kind": seq_kind, "sequential_other_wall_s": seq_other,
         "sequential_sample": "first %d requests, one at a time, FP64" % nseq,
         "vs_sequential": round(rate / seq_rate, 3),
         "energy_error_max_Eh": dE, "energy_error_jobs": ndE,
@@ -317,18 +333,26 @@ def main():
     ap.add_argument("--workload", choices=["c60", "batch", "serve"], default="c60")
     ap.add_argument("--chunk", type=int, default=0, help="quartets per block-class chunk (0: library default)")
     ap.add_argument("--batch-n", type=int, default=512)
-    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--workers", type=int, default=None,
+                    help="batch/serve: groups (batched path) or jobs (per-job path) in flight")
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--seq-n", type=int, default=32, help="batch/serve: jobs of the sequential FP64 baseline")
     ap.add_argument("--scf", choices=["host", "device"], default="host",
-                    help="batch/serve: SCF driver of each job (host: scf.rhf, device: hfx_scf_run)")
+                    help="batch/serve per-job path: SCF driver of each job (host: scf.rhf, device: hfx_scf_run)")
+    ap.add_argument("--path", choices=["batched", "per-job"], default="batched",
+                    help="batch/serve: shared launches in batch contexts, or one context per job")
+    ap.add_argument("--group", type=int, default=64, help="batch/serve: molecules per batch context, at most")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the build would shard its screened quartet lists by cost and all-reduce X once per build)",
               file=sys.stderr)
         sys.exit(2)
+    if a.workers is None:
+        a.workers = (2 if a.workload == "batch" else 4) if a.path == "batched" else 16
+    if a.workload == "serve" and a.group == 64 and a.path == "batched":
+        a.group = 32
     if a.workload == "batch":
         return run_batch(a)
     if a.workload == "c60" and a.split is None:

@@ -1,5 +1,15 @@
 """Many-molecule jobs on one GPU: batched SCF solves and a serving loop.
 
+Shared launches (the default): molecules are grouped by basis size; each
+group is one batch context (``fock.BatchBuilder``: every quartet class of
+every molecule of the group in one launch of its kernels) and one batched
+SCF on the GPU (``BatchBuilder.scf``: all of the group's S, H, P, F, DIIS
+history and orbitals resident in device memory, each iteration's build and
+post-build sequence replayed from HIP graphs).  A few groups run at once on
+their own streams.  ``solve_group`` / ``run_batched`` / ``BatchServer``.
+
+Per job (kept as the sequential baseline and for comparison):
+
 Each job is a full RHF solve (the driver of direct_hartreefock.f90:98-223:
 ``hfcode_amd.scf.rhf`` on the host around the GPU builds, or with
 ``device_scf`` all of it on the GPU through ``FockBuilder.scf``) with its own
@@ -217,3 +227,191 @@ def latency_summary(results):
     lat = np.array([r["latency_s"] for r in results])
     return dict(n=len(lat), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
                 mean_s=float(lat.mean()), max_s=float(lat.max()))
+
+
+# ---- shared launches: groups of molecules in batch contexts -------------------
+
+def _nb_of(mol, basis):
+    return M.ShellTable(mol, basis).nb
+
+
+def solve_group(mols, basis_name, maxiter=50, diis_start=DIIS_START, split=None, use_graphs=True, on_done=None):
+    """RHF of every molecule of `mols` (one basis) in one batch context and
+    one batched device SCF; returns result dicts in input order.
+    `on_done(i, result)` is called as molecule i finishes (converged or not).
+    Guess and DIIS schedule as :func:`solve`; `split` as FockBuilder.set_mixed
+    (full builds only; incremental builds stay FP64)."""
+    t0 = time.perf_counter()
+    basis = _basis(basis_name)
+    sts = [M.ShellTable(m, basis) for m in mols]
+    bb = fock.BatchBuilder(sts)
+    out = [None] * len(mols)
+    try:
+        if split:
+            bb.set_mixed(split)
+        S, T, V = bb.onee(mols)
+        H = T + V
+        P0 = bb.stack([M.atomic_density_guess(m, basis) for m in mols]) if basis.get("density") else None
+        t1 = time.perf_counter()
+
+        def done(i, r):
+            out[i] = dict(name=mols[i].title, natom=mols[i].natom, nb=sts[i].nb, basis=basis_name, energy=r.energy,
+                          iterations=r.iterations, converged=r.converged, status=r.status, split=split,
+                          setup_s=t1 - t0, scf_s=time.perf_counter() - t1, group=len(mols))
+            if on_done is not None:
+                on_done(i, out[i])
+        bb.scf(S, H, [m.nocc for m in mols], [m.nuclear_repulsion() for m in mols], P0=P0, maxiter=maxiter,
+               diis_start=diis_start, use_graphs=use_graphs, on_done=done)
+    finally:
+        bb.close()
+    return out
+
+
+def group_by_size(nbs, group):
+    """Index groups of at most `group` molecules, consecutive in basis size
+    (so that a group's stacked matrices carry little padding)."""
+    order = sorted(range(len(nbs)), key=lambda i: nbs[i])
+    return [order[k:k + group] for k in range(0, len(order), group)]
+
+
+def run_batched(mols, basis_name, group=64, workers=2, maxiter=50, split=None, use_graphs=True, progress=None):
+    """Solve every molecule of `mols` in size-sorted groups of `group`
+    molecules, `workers` groups at once; returns (results in input order,
+    wall seconds)."""
+    fock.lib()
+    basis = _basis(basis_name)
+    nbs = [_nb_of(m, basis) for m in mols]
+    out = [None] * len(mols)
+    lock = threading.Lock()
+    count = [0]
+    t0 = time.perf_counter()
+
+    def one(idx):
+        def done(i, r):
+            with lock:
+                out[idx[i]] = r
+                count[0] += 1
+                if progress is not None:
+                    progress(count[0], len(mols), time.perf_counter() - t0)
+        solve_group([mols[i] for i in idx], basis_name, maxiter=maxiter, split=split, use_graphs=use_graphs,
+                    on_done=done)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        for f in [ex.submit(one, g) for g in group_by_size(nbs, group)]:
+            f.result()
+    return out, time.perf_counter() - t0
+
+
+def group_memory_estimate(sts):
+    """Device bytes a batch context and its batched SCF may take for shell
+    tables `sts`: the quartet lists and integral buffer of every molecule
+    (memory_estimate) and the SCF's stacked matrices ((15 + 2 ndiis) per
+    molecule at the group's largest nb)."""
+    nb = max(st.nb for st in sts)
+    return sum(memory_estimate(st) for st in sts) + 8 * 31 * nb * nb * len(sts)
+
+
+class BatchServer:
+    """Serving loop with continuous batching.  Requests (arrival time,
+    molecule, basis) queue as they arrive; whenever one of `workers` slots is
+    free, the queued requests of one basis (oldest first, at most `group`,
+    within the memory budget next to the running groups) start as one
+    micro-batch: one batch context and one batched SCF, so their builds share
+    every launch.  Micro-batches overlap on the GPU on their own streams; a
+    request's result is reported as soon as its own SCF finishes, while the
+    rest of its micro-batch goes on.  Latency = completion - arrival."""
+
+    def __init__(self, workers=4, group=32, mem_budget=200 * (1 << 30), maxiter=50, split=None, use_graphs=True):
+        self.workers = workers
+        self.group = group
+        self.mem_budget = mem_budget
+        self.maxiter = maxiter
+        self.split = split
+        self.use_graphs = use_graphs
+
+    def run(self, requests, progress=None):
+        """Replay `requests` in real time; returns (results sorted by arrival, wall seconds)."""
+        fock.lib()
+        cond = threading.Condition()
+        queue = []                  # (index, arrival, molecule, basis, shell table)
+        out = []
+        state = dict(running=0, mem=0, arrived=0)
+        t_start = time.perf_counter()
+
+        def worker_run(batch, need):
+            idx = [b[0] for b in batch]
+            mols = [b[2] for b in batch]
+            basis = batch[0][3]
+
+            def done(i, r):
+                t = time.perf_counter() - t_start
+                r.update(req=idx[i], arrival_s=batch[i][1], done_s=t, latency_s=t - batch[i][1], error=None)
+                with cond:
+                    out.append(r)
+                    if progress is not None:
+                        progress(len(out), len(requests), t)
+            try:
+                solve_group(mols, basis, maxiter=self.maxiter, split=self.split, use_graphs=self.use_graphs,
+                            on_done=done)
+            except Exception as e:   # a failed micro-batch is reported, the loop goes on
+                t = time.perf_counter() - t_start
+                with cond:
+                    for k in range(len(batch)):
+                        if not any(o.get("req") == idx[k] for o in out):
+                            out.append(dict(req=idx[k], name=mols[k].title, natom=mols[k].natom, basis=basis, error=repr(e),
+                                            converged=False, arrival_s=batch[k][1], done_s=t,
+                                            latency_s=t - batch[k][1]))
+            with cond:
+                state["running"] -= 1
+                state["mem"] -= need
+                cond.notify_all()
+
+        def dispatcher():
+            threads = []
+            while True:
+                with cond:
+                    while True:
+                        if not queue and state["arrived"] == len(requests):
+                            for th in threads:
+                                th.join(0)
+                            return threads
+                        if queue and state["running"] < self.workers:
+                            basis = queue[0][3]
+                            batch, need = [], 0
+                            for q in list(queue):
+                                if q[3] != basis or len(batch) >= self.group:
+                                    continue
+                                est = group_memory_estimate([b[4] for b in batch] + [q[4]])
+                                if batch and state["mem"] + est > self.mem_budget:
+                                    break
+                                batch.append(q)
+                                need = est
+                            if batch and (state["running"] == 0 or state["mem"] + need <= self.mem_budget):
+                                for q in batch:
+                                    queue.remove(q)
+                                state["running"] += 1
+                                state["mem"] += need
+                                break
+                        cond.wait(0.05)
+                th = threading.Thread(target=worker_run, args=(batch, need), daemon=True)
+                th.start()
+                threads.append(th)
+
+        disp_threads = []
+        disp = threading.Thread(target=lambda: disp_threads.extend(dispatcher()), daemon=True)
+        with _blas_one_thread():
+            disp.start()
+            for i, (arrival, mol, basis) in enumerate(requests):
+                now = time.perf_counter() - t_start
+                if arrival > now:
+                    time.sleep(arrival - now)
+                st = M.ShellTable(mol, _basis(basis))
+                with cond:
+                    queue.append((i, arrival, mol, basis, st))
+                    state["arrived"] += 1
+                    cond.notify_all()
+            disp.join()
+            for th in disp_threads:
+                th.join()
+            wall = time.perf_counter() - t_start
+        out.sort(key=lambda r: r["arrival_s"])
+        return out, wall

@@ -32,7 +32,7 @@ HIPFLAGS = [
 
 
 def sources(gen=GEN):
-    srcs = [os.path.join(CSRC, f) for f in ("hfx_kernels.hip", "hfx_host.hip", "hfx_scf.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("hfx_kernels.hip", "hfx_host.hip", "hfx_scf.hip", "hfx_batch.hip")]
     srcs += sorted(os.path.join(gen, f) for f in os.listdir(gen) if f.endswith(".hip"))
     return srcs
 

@@ -17,15 +17,21 @@
 namespace hfx {
 
 // One shell pair, in the orientation of the reference pair lists
-// (precalc_kabs.f90:38-90): shell a is the first listed shell.
+// (precalc_kabs.f90:38-90): shell a is the first listed shell.  A context
+// may hold several molecules (a batch): their shells are numbered one after
+// the other, AO offsets are the molecule's own, and each molecule's nb x nb
+// matrices (P, X, F) are stacked, molecule m's block at m * nb * nb with nb
+// the context's row stride.
 struct PairRec {
   double ax, ay, az, bx, by, bz;   // centres (bohr)
-  int sha, shb;                    // shells, 0-based
-  int aoa, aob;                    // first AO, 0-based
+  int sha, shb;                    // shells, 0-based, numbered across the context's molecules
+  int aoa, aob;                    // first AO in the molecule, 0-based
   int p0;                          // first primitive pair record
   int np;                          // primitive pairs that pass the negab test
   int npall;                       // all primitive pairs (negab ones follow)
-  int rank;                        // position in the Schwarz order of its type
+  int mol;                         // molecule of the pair (0 with one molecule)
+  int lsa, lsb;                    // shells a, b numbered within the molecule
+  long long mbase;                 // first element of the molecule's block of P, X and F
 };
 
 // One primitive pair (precalc_kabs.f90:94-115): p = za + zb, 1/p, the
@@ -65,6 +71,7 @@ struct ClassArgs {
   int lds_rows;            // fused kernels: 1 = rows of X of the resident bra pair in LDS
   int rows_offset;         // fused kernels: byte offset of those rows in the dynamic LDS
   double* pairbuf;         // fused kernels: J_cd per ket pair (HFX_PAIRBUF_STRIDE doubles per pair)
+  const double* thr_mol;   // block test threshold per molecule (device), or null: thr for every quartet
 };
 
 // Resident bra pair for a step [lo, hi) of a bra-contiguous quartet list

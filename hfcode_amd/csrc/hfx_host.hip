@@ -123,10 +123,12 @@ void release_cached(int device) {   // device < 0: every device
       }
     }
   }
+  hfx::DeviceSection ds;
   for (void* p : drop) hipFree(p);
 }
 
 void* dev_alloc(size_t bytes) {
+  if (hfx::t_capturing) return nullptr;   // never inside a capture section
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   const std::pair<int, size_t> key(dev, size_class(bytes));
@@ -142,9 +144,15 @@ void* dev_alloc(size_t bytes) {
     }
   }
   void* p = nullptr;
-  if (hipMalloc(&p, key.second) != hipSuccess) {
+  hipError_t e;
+  {
+    hfx::DeviceSection ds;
+    e = hipMalloc(&p, key.second);
+  }
+  if (e != hipSuccess) {
     (void)hipGetLastError();
     release_cached(dev);            // memory held by the cache, then try once more
+    hfx::DeviceSection ds;
     if (hipMalloc(&p, key.second) != hipSuccess) {
       (void)hipGetLastError();
       return nullptr;
@@ -168,15 +176,29 @@ void dev_free(void* p) {
       return;
     }
   }
+  hfx::DeviceSection ds;
   hipFree(p);
 }
 
 }  // namespace
 
+std::shared_mutex& hfx::capture_mutex() {
+  static std::shared_mutex* m = new std::shared_mutex;   // lives until the process ends
+  return *m;
+}
+thread_local bool hfx::t_capturing = false;
+
 void* hfx::cached_alloc(size_t bytes) { return dev_alloc(bytes); }
 void hfx::cached_free(void* p) { dev_free(p); }
 
 namespace {
+
+// A copy in order on `st` (never the null stream: see hfx::capture_mutex),
+// waited for.
+hipError_t copy_sync(hipStream_t st, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+  return e != hipSuccess ? e : hipStreamSynchronize(st);
+}
 
 template <typename T>
 int dalloc(T** p, size_t n) {
@@ -202,16 +224,54 @@ struct ClassPlan {
   double* blocks32 = nullptr;
 };
 
-struct hfx_ctx {
-  int nb = 0, nsh = 0, dev = 0, ncu = 0;
-  std::vector<Shell> shells;
+// One molecule of a context.  A context holds one molecule (hfx_ctx_create)
+// or several whose builds share every launch (hfx_batch_create): shells are
+// numbered across the molecules, pairs of a type are grouped by molecule,
+// and molecule m's nb x nb matrices start at m * nb * nb (nb = the context's
+// stride, the largest molecule's basis size).
+struct MolPart {
+  int nb = 0, nsh = 0, sh0 = 0;                 // basis functions, shells, first shell
   int npair[HFX_NPAIRTYPES] = {0};
-  int offset[HFX_NPAIRTYPES] = {0};
-  std::vector<int> prs[HFX_NPAIRTYPES];         // 2 per pair, 0-based shells
-  std::vector<double> schw[HFX_NPAIRTYPES];     // per pair (list order)
-  std::vector<int> order[HFX_NPAIRTYPES];       // 0-based
+  int pbeg[HFX_NPAIRTYPES] = {0};               // first global pair of the molecule's type-t list
+  std::vector<double> schw[HFX_NPAIRTYPES];     // per pair (the molecule's list order)
+  std::vector<int> order[HFX_NPAIRTYPES];       // 0-based positions in that list
   int last[HFX_NPAIRTYPES] = {0};
+};
+
+// A build captured in a HIP graph, replayed by later calls with the same arguments.
+struct GraphKey {
+  const double* P;
+  double* F;
+  const double* thr_diag;
+  const double* thr_nd;
+  const double* split_mol;
+  int mode, zero_f;
+  double split;
+  bool operator==(const GraphKey& o) const {
+    return P == o.P && F == o.F && thr_diag == o.thr_diag && thr_nd == o.thr_nd && split_mol == o.split_mol &&
+           mode == o.mode && zero_f == o.zero_f && split == o.split;
+  }
+};
+struct CachedGraph {
+  GraphKey key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+struct hfx_ctx {
+  int nb = 0, nsh = 0, dev = 0, ncu = 0;        // nb: matrix stride; nsh: shells of all molecules
+  int nmol = 1, nsp = 0;                        // molecules; nsp: largest shell count (maxdm block)
+  std::vector<MolPart> mols;
+  std::vector<Shell> shells;
+  int npair[HFX_NPAIRTYPES] = {0};              // pairs of the type over all molecules
+  int offset[HFX_NPAIRTYPES] = {0};             // first global pair of the type
+  int act_len[HFX_NPAIRTYPES] = {0};            // active (truncated) list length over all molecules
+  int nchunk_max[HFX_NPAIRTYPES] = {0};         // largest per-molecule active length, in waves
   bool have_schwarz = false;
+  bool device_chunks = false;     // block classes chunked by capacity: no count read-back (graph capture)
+  bool capturing = false;         // a HIP graph is being captured from the context's stream
+  std::vector<CachedGraph> graphs;  // captured builds; dropped whenever a buffer they use is replaced
+  bool graphs_off = false;          // a capture failed: later builds run uncaptured
   std::vector<PairRec> pairs;
   std::vector<PrimRec> prims;
   std::vector<ClassPlan> classes;
@@ -252,7 +312,10 @@ struct hfx_ctx {
   double* d_F = nullptr;
   double* d_X = nullptr;
   double* d_maxdm = nullptr;
-  unsigned long long* d_pmax = nullptr;        // max over maxdm (bits of the double)
+  unsigned long long* d_pmax = nullptr;        // per molecule: max over maxdm (bits of the double)
+  int2* d_molsh = nullptr;                     // per molecule: first shell, shell count
+  int* d_bmol[HFX_NPAIRTYPES] = {nullptr};       // molecule per active position
+  int* d_aseg[HFX_NPAIRTYPES] = {nullptr};       // first active position per molecule (nmol + 1)
   unsigned long long* d_masks = nullptr;       // screening survivor bits of one class
   size_t masks_words = 0;
   int act_sorted[HFX_NPAIRTYPES] = {0};         // active Schwarz values non-increasing
@@ -264,7 +327,7 @@ struct hfx_ctx {
   int2* d_diag_list[HFX_NPAIRTYPES] = {nullptr};
   // one-electron set-up on the device (allocated on first use)
   int4* d_onee_items = nullptr;
-  int onee_count[3][3] = {};                   // work items per (la, lb), consecutive in d_onee_items
+  std::vector<int> onee_count;                 // work items per (molecule, la, lb), consecutive in d_onee_items
   int *d_sh_type = nullptr, *d_sh_np = nullptr, *d_sh_ao = nullptr;
   double *d_sh_xyz = nullptr, *d_sh_z = nullptr, *d_sh_c = nullptr, *d_sh_cl = nullptr;
   int* d_counts = nullptr;                     // [2][HFX_NCLASSES]
@@ -302,47 +365,87 @@ hipError_t sync_all_streams(hfx_ctx* c) {
   return e;
 }
 
-// Active (Schwarz-truncated) lists on the device, from schw/order/last.
+// Captured builds hold the addresses of the context's buffers: they are
+// dropped (and captured again on their next call) whenever one is replaced.
+void drop_graphs(hfx_ctx* c) {
+  if (c->graphs.empty()) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  hfx::DeviceSection ds;
+  for (CachedGraph& g : c->graphs) {
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    if (g.graph) hipGraphDestroy(g.graph);
+  }
+  c->graphs.clear();
+}
+
+// Active (Schwarz-truncated) lists on the device, from each molecule's
+// schw/order/last: per pair type, the molecules' truncated lists one after
+// the other (each in its own Schwarz order), with the molecule of every
+// position and where each molecule's segment starts.
 int upload_active(hfx_ctx* c) {
   HIPOK(sync_all_streams(c));   // the lists below are replaced; no build may still read them
+  drop_graphs(c);
   for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
-    const int n = c->npair[t];
-    if (n == 0) continue;
-    std::vector<int> act(c->last[t]);
-    std::vector<double> asc(c->last[t]);
-    std::vector<int4> info(c->last[t]);
-    for (int k = 0; k < c->last[t]; ++k) {
-      const int pi = c->order[t][k];
-      act[k] = c->offset[t] + pi;
-      asc[k] = c->schw[t][pi];
-      PairRec& pr = c->pairs[act[k]];
-      pr.rank = k;
-      info[k] = make_int4(pr.sha, pr.shb, pr.np, pr.npall);
+    if (c->npair[t] == 0) continue;
+    std::vector<int> act, bmol, aseg(c->nmol + 1, 0);
+    std::vector<double> asc;
+    std::vector<int4> info;
+    int sorted = 1, maxlen = 0;
+    for (int m = 0; m < c->nmol; ++m) {
+      const MolPart& mp = c->mols[m];
+      aseg[m] = (int)act.size();
+      maxlen = std::max(maxlen, mp.last[t]);
+      for (int k = 0; k < mp.last[t]; ++k) {
+        const int pi = mp.order[t][k];
+        const int g = mp.pbeg[t] + pi;
+        act.push_back(g);
+        asc.push_back(mp.schw[t][pi]);
+        if (k > 0 && asc.back() > asc[asc.size() - 2]) sorted = 0;
+        const PairRec& pr = c->pairs[g];
+        info.push_back(make_int4(pr.lsa, pr.lsb, pr.np, pr.npall));
+        bmol.push_back(m);
+      }
     }
-    c->act_sorted[t] = 1;
-    for (int k = 1; k < c->last[t]; ++k)
-      if (asc[k] > asc[k - 1]) c->act_sorted[t] = 0;
+    aseg[c->nmol] = (int)act.size();
+    c->act_len[t] = (int)act.size();
+    c->nchunk_max[t] = (maxlen + HFX_WAVE - 1) / HFX_WAVE;
+    c->act_sorted[t] = sorted;
     dfree(c->d_act[t]);
     dfree(c->d_act_schw[t]);
     dfree(c->d_act_info[t]);
+    dfree(c->d_bmol[t]);
+    dfree(c->d_aseg[t]);
     if (dalloc(&c->d_act[t], act.size()) || dalloc(&c->d_act_schw[t], asc.size()) ||
-        dalloc(&c->d_act_info[t], info.size()))
+        dalloc(&c->d_act_info[t], info.size()) || dalloc(&c->d_bmol[t], bmol.size()) ||
+        dalloc(&c->d_aseg[t], aseg.size()))
       return HFX_E_NOMEM;
-    HIPOK(hipMemcpy(c->d_act_info[t], info.data(), info.size() * sizeof(int4), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(c->d_act[t], act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(c->d_act_schw[t], asc.data(), asc.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(c->d_pair_schw[t], c->schw[t].data(), n * sizeof(double), hipMemcpyHostToDevice));
+    if (!act.empty()) {
+      HIPOK(copy_sync(c->stream, c->d_act_info[t], info.data(), info.size() * sizeof(int4), hipMemcpyHostToDevice));
+      HIPOK(copy_sync(c->stream, c->d_act[t], act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice));
+      HIPOK(copy_sync(c->stream, c->d_act_schw[t], asc.data(), asc.size() * sizeof(double), hipMemcpyHostToDevice));
+      HIPOK(copy_sync(c->stream, c->d_bmol[t], bmol.data(), bmol.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    HIPOK(copy_sync(c->stream, c->d_aseg[t], aseg.data(), aseg.size() * sizeof(int), hipMemcpyHostToDevice));
+    std::vector<double> ps(c->npair[t]);
+    for (int m = 0; m < c->nmol; ++m) {
+      const MolPart& mp = c->mols[m];
+      for (int k = 0; k < mp.npair[t]; ++k) ps[mp.pbeg[t] - c->offset[t] + k] = mp.schw[t][k];
+    }
+    HIPOK(copy_sync(c->stream, c->d_pair_schw[t], ps.data(), ps.size() * sizeof(double), hipMemcpyHostToDevice));
   }
-  HIPOK(hipMemcpy(c->d_pairs, c->pairs.data(), c->pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice));
-  // class plans and their work lists
+  // class plans and their work lists: capacity = candidate quartets of every molecule
   size_t need_masks = 0;
   for (auto& cp : c->classes) {
     dfree(cp.list64);
     dfree(cp.list32);
-    const long long nb = c->last[cp.t1], nk = c->last[cp.t2];
-    cp.cap = cp.same ? nb * (nb - 1) / 2 : nb * nk;
+    long long cap = 0;
+    for (int m = 0; m < c->nmol; ++m) {
+      const long long nb = c->mols[m].last[cp.t1], nk = c->mols[m].last[cp.t2];
+      cap += cp.same ? nb * (nb - 1) / 2 : nb * nk;
+    }
+    cp.cap = cap;
     if (cp.cap > 0 && dalloc(&cp.list64, (size_t)cp.cap)) return HFX_E_NOMEM;
-    need_masks = std::max(need_masks, (size_t)nb * (size_t)((nk + HFX_WAVE - 1) / HFX_WAVE) * 2);
+    need_masks = std::max(need_masks, (size_t)c->act_len[cp.t1] * (size_t)c->nchunk_max[cp.t2] * 2);
   }
   if (need_masks > c->masks_words) {
     dfree(c->d_masks);
@@ -354,7 +457,7 @@ int upload_active(hfx_ctx* c) {
 }
 
 int ensure_mixed_lists(hfx_ctx* c) {
-  if (!c->d_P32 && dalloc(&c->d_P32, (size_t)c->nb * c->nb)) return HFX_E_NOMEM;
+  if (!c->d_P32 && dalloc(&c->d_P32, (size_t)c->nb * c->nb * c->nmol)) return HFX_E_NOMEM;
   for (auto& cp : c->classes)
     if (!cp.list32 && cp.cap > 0 && dalloc(&cp.list32, (size_t)cp.cap)) return HFX_E_NOMEM;
   return HFX_OK;
@@ -377,7 +480,8 @@ ClassArgs base_args(hfx_ctx* c, const double* dP, double thr) {
 
 int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long long cap) {
   const ClassKernels& k = kClassKernels[cls];
-  if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
+  const bool prof = c->profile && !c->device_chunks;
+  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
   const void* fn = prec == 1 ? k.f32 : (jonly ? k.f64j : k.f64);
   const int K = prec == 1 ? 5 : 8;
   const size_t boys_shm = (size_t)HFX_BOYS_ROWS * (k.M + K) * (prec == 1 ? sizeof(float) : sizeof(double));
@@ -393,7 +497,7 @@ int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   const int grid = grid_for(cap, HFX_NT, c->ncu * per_cu);
   void* args[] = {&a};
   HIPOK(hipLaunchKernel(fn, dim3(grid), dim3(HFX_NT), args, shm, c->stream));
-  if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));
+  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));
   c->class_ran[prec][cls] = 1;
   return HFX_OK;
 }
@@ -422,7 +526,12 @@ int ensure_side_streams(hfx_ctx* c) {
 
 // Heavy classes: combination kernels (integrals to HBM, tiles of 64
 // quartets) + the class's digestion kernel, chunk by chunk over quartets
-// [0, n).  The digestion walks contiguous ranges of the (bra-contiguous)
+// [0, n).  n is the list's length read back from the device, or its
+// capacity when the context sizes its launches on the device
+// (device_chunks): chunks past the list's end then exit at once, and the
+// whole build is one fixed launch sequence on the context's stream (no side
+// streams, no buffer growth, no host synchronisation), which a HIP graph can
+// capture.  The digestion walks contiguous ranges of the (bra-contiguous)
 // list and, where the workgroup's LDS holds the rows of X that a bra pair's
 // quartets update (rows of shells A and B: J_ab, K_ac, K_ad, K_bc, K_bd),
 // accumulates them there and flushes them with row-coalesced atomics.
@@ -439,9 +548,11 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   const long long limit = live_contexts.load() == 1 ? kEblkDoublesAlone : kEblkDoubles;
   const long long need = std::min(limit, ((n + HFX_NT - 1) / HFX_NT) * HFX_NT * ne);
   if (need > c->eblk_doubles) {
+    if (c->capturing) return HFX_E_STATE;   // buffers are sized by an uncaptured build first
     const long long cap = std::min(limit, std::max(need, 2 * c->eblk_doubles));
     const long long rows = std::min(kMaxChunk, (cap / 16 + HFX_NT - 1) / HFX_NT * HFX_NT);
     HIPOK(sync_all_streams(c));   // earlier chunks may still use the old buffers
+    drop_graphs(c);
     dfree(c->d_eblk);
     dfree(c->d_vmax);
     c->eblk_doubles = c->vmax_rows = 0;
@@ -459,7 +570,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   const long long half_e = c->eblk_doubles / 2 / 64 * 64;   // doubles per half, 512-byte aligned
   const long long half_v = c->vmax_rows / 2 * kMaxCombo;    // vmax doubles per half
   const long long half_chunk = std::min(c->vmax_rows / 2, half_e / ne) / HFX_NT * HFX_NT;
-  const bool pipelined = n > chunk && half_chunk >= HFX_NT && live_contexts.load() == 1;
+  const bool pipelined = n > chunk && half_chunk >= HFX_NT && live_contexts.load() == 1 && !c->device_chunks;
   if (pipelined) {
     chunk = std::min(chunk, half_chunk);
     int r = ensure_side_streams(c);
@@ -473,7 +584,8 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   const bool rows = !a.diag && rows_shm + tb_shm <= c->max_shm;
   const size_t dig_shm = (rows ? rows_shm : 0) + tb_shm;
   const int ncombo = prec == 1 ? k.ncombo32 : k.ncombo;
-  if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
+  const bool prof = c->profile && !c->device_chunks;
+  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
   if (pipelined) {
     // the side streams start after everything already on the context's stream
     HIPOK(hipEventRecord(c->fork, c->stream));
@@ -502,7 +614,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
       nside = std::min(ncombo, (int)hfx_ctx::kSide);
       if (j >= 2)
         for (int s = 0; s < nside; ++s) HIPOK(hipStreamWaitEvent(c->side[s], c->done[h], 0));
-    } else if (ncombo > 1 && q1 - q0 >= kConcurrentChunk && live_contexts.load() == 1) {
+    } else if (ncombo > 1 && q1 - q0 >= kConcurrentChunk && live_contexts.load() == 1 && !c->device_chunks) {
       nside = std::min(ncombo - 1, (int)hfx_ctx::kSide);
       int r = ensure_side_streams(c);
       if (r) return r;
@@ -557,6 +669,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     for (int i = 0; i < 4; ++i) g.dims[i] = k.dims[i];
     g.swap_bra = k.swap_bra;
     g.swap_ket = k.swap_ket;
+    g.thr_mol = a.thr_mol;
     void* gargs[] = {&g};
     // FP32 quartets: blocks stored as floats and contracted with the FP32 copy
     // of P; every quartet's sums are added to X in FP64
@@ -577,28 +690,30 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     HIPOK(hipLaunchKernel(dfn, dim3(grid), dim3(HFX_NT), gargs, dig_shm, c->stream));
     if (pipelined) HIPOK(hipEventRecord(c->done[h], c->stream));
   }
-  if (c->profile) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));
+  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));
   c->class_ran[prec][cls] = 1;
   return HFX_OK;
 }
 
 int zero_build_state(hfx_ctx* c) {
   memset(c->class_ran, 0, sizeof(c->class_ran));
-  const size_t nx = (size_t)c->nb * c->nb;
+  const size_t nx = (size_t)c->nb * c->nb * c->nmol;
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)nx, 256, 2048)), dim3(256), 0, c->stream, c->d_X, nx, 0.0);
   const size_t npb = (size_t)c->totpairs * HFX_PAIRBUF_STRIDE;
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)npb, 256, 2048)), dim3(256), 0, c->stream, c->d_pairbuf, npb,
                      0.0);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, c->stream, c->d_counts, 2 * HFX_NCLASSES, 0);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, c->stream, c->d_stats, 2 * HFX_NCLASSES * 3);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->d_pmax, 1);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(c->nmol, 256, 64)), dim3(256), 0, c->stream, c->d_pmax, c->nmol);
   HIPOK(hipGetLastError());
   return HFX_OK;
 }
 
 int launch_maxdm(hfx_ctx* c, const double* dP) {
-  const long long n = (long long)c->nsh * c->nsh;
-  hipLaunchKernelGGL(k_maxdm, dim3(grid_for(n, 256, 4096)), dim3(256), 0, c->stream, dP, c->nb, c->d_shells, c->nsh, c->d_maxdm, c->d_pmax);
+  const long long n = (long long)c->nsp * c->nsp;
+  const dim3 grid(grid_for(n, 256, std::max(1, 4096 / c->nmol)), c->nmol);
+  hipLaunchKernelGGL(k_maxdm, grid, dim3(256), 0, c->stream, dP, c->nb, c->d_shells, c->d_molsh, c->nsp, c->d_maxdm,
+                     c->d_pmax);
   HIPOK(hipGetLastError());
   return HFX_OK;
 }
@@ -619,21 +734,23 @@ int alloc_debug_blocks(hfx_ctx* c, ClassPlan& cp, long long cap, int prec) {
 }
 
 // Off-diagonal build on device arrays: F += G(P); quartets whose bound is
-// below `split` (> 0) in FP32.
-int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, double split) {
+// below `split` (> 0) in FP32.  thr_mol (device, one per molecule) replaces
+// thr when given.  P and F hold the context's nmol stacked matrices.
+int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, double split,
+             const double* thr_mol = nullptr, const double* split_mol = nullptr) {
   if (!c->have_schwarz) return HFX_E_STATE;
   const bool jonly = mode == HFX_J_ONLY;
-  const bool mixed = split > 0.0 && !jonly;
+  const bool mixed = (split > 0.0 || split_mol) && !jonly;   // split_mol: the FP32 split per molecule
   if (mixed) {
     int r = ensure_mixed_lists(c);
     if (r) return r;
   }
   int r;
-  HIPOK(hipEventRecord(c->ev0, c->stream));
+  if (!c->capturing) HIPOK(hipEventRecord(c->ev0, c->stream));
   if ((r = zero_build_state(c))) return r;
   if ((r = launch_maxdm(c, dP))) return r;
   if (mixed) {   // the FP32 quartets are contracted with an FP32 copy of P
-    const size_t nb2 = (size_t)c->nb * c->nb;
+    const size_t nb2 = (size_t)c->nb * c->nb * c->nmol;
     hipLaunchKernelGGL(k_to_f32, dim3(grid_for((long long)nb2, 256, 2048)), dim3(256), 0, c->stream, dP, c->d_P32, nb2);
     HIPOK(hipGetLastError());
   }
@@ -649,6 +766,11 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
     s.schw_ket = c->d_act_schw[cp.t2];
     s.pairs = c->d_pairs;
     s.maxdm = c->d_maxdm;
+    s.md_stride = (long long)c->nsp * c->nsp;
+    s.bmol = c->nmol > 1 ? c->d_bmol[cp.t1] : nullptr;
+    s.kseg = c->nmol > 1 ? c->d_aseg[cp.t2] : nullptr;
+    s.thr_mol = thr_mol;
+    s.split_mol = mixed ? split_mol : nullptr;
     s.list64 = cp.list64;
     s.list32 = mixed ? cp.list32 : nullptr;
     s.count64 = c->d_counts + cp.cls;
@@ -657,15 +779,15 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
     s.stats32 = c->d_stats + 3 * (HFX_NCLASSES + cp.cls);
     s.thr = thr;
     s.split = mixed ? split : 0.0;
-    s.nbra = c->last[cp.t1];
-    s.nket = c->last[cp.t2];
+    s.nbra = c->act_len[cp.t1];
+    s.nket = c->act_len[cp.t2];
     s.same = cp.same;
     s.jonly = jonly;
-    s.nsh = c->nsh;
+    s.nsh = c->nsp;
     s.all_prims = kClassKernels[cp.cls].all_prims;
     s.pmax = c->d_pmax;
     s.masks = c->d_masks;
-    s.nchunk = (s.nket + HFX_WAVE - 1) / HFX_WAVE;
+    s.nchunk = c->nchunk_max[cp.t2];
     s.ket_sorted = c->act_sorted[cp.t2];
     // one wave per bra position
     hipLaunchKernelGGL(k_screen, dim3(grid_for(s.nbra, 256 / HFX_WAVE, c->ncu * 16)), dim3(256), 0, c->stream, s);
@@ -676,7 +798,7 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
   bool any_split = false;
   for (auto& cp : c->classes)
     if (cp.cap > 0 && kClassKernels[cp.cls].split) any_split = true;
-  if (any_split) {
+  if (any_split && !c->device_chunks) {
     HIPOK(hipMemcpyAsync(hcounts, c->d_counts, sizeof(hcounts), hipMemcpyDeviceToHost, c->stream));
     HIPOK(hipStreamSynchronize(c->stream));
   }
@@ -684,6 +806,7 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
     if (cp.cap <= 0) continue;
     for (int prec = 0; prec < (mixed ? 2 : 1); ++prec) {
       ClassArgs a = base_args(c, dP, thr);
+      a.thr_mol = thr_mol;
       a.list = prec ? cp.list32 : cp.list64;
       a.count = c->d_counts + prec * HFX_NCLASSES + cp.cls;
       a.diag = 0;
@@ -692,7 +815,8 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
         a.blocks = prec ? cp.blocks32 : cp.blocks64;
       }
       if (kClassKernels[cp.cls].split)
-        r = launch_split(c, cp.cls, prec, jonly, a, hcounts[prec * HFX_NCLASSES + cp.cls], a.blocks);
+        r = launch_split(c, cp.cls, prec, jonly, a,
+                         c->device_chunks ? cp.cap : hcounts[prec * HFX_NCLASSES + cp.cls], a.blocks);
       else
         r = launch_class(c, cp.cls, prec, jonly, a, cp.cap);
       if (r) return r;
@@ -700,18 +824,20 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
   }
   hipLaunchKernelGGL(k_pairbuf_to_x, dim3(grid_for((long long)c->totpairs * HFX_PAIRBUF_STRIDE, 256, 2048)), dim3(256), 0,
                      c->stream, c->d_pairbuf, c->d_pairs, c->d_shells, c->totpairs, c->d_X, c->nb);
-  hipLaunchKernelGGL(k_add_sym, dim3(grid_for((long long)c->nb * c->nb, 256, 2048)), dim3(256), 0, c->stream, c->d_X, dF, c->nb);
+  hipLaunchKernelGGL(k_add_sym, dim3(grid_for((long long)c->nb * c->nb * c->nmol, 256, 4096)), dim3(256), 0, c->stream,
+                     c->d_X, dF, c->nb, c->nmol);
   HIPOK(hipGetLastError());
-  HIPOK(hipEventRecord(c->ev1, c->stream));
+  if (!c->capturing) HIPOK(hipEventRecord(c->ev1, c->stream));
   c->last_kind = 1;
   return HFX_OK;
 }
 
 // Diagonal-pair build (fock_build_diag, or fock_build_initial_diag when
-// initial != 0: no density test, Schwarz values written out).
-int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initial) {
+// initial != 0: no density test, Schwarz values written out); thr_mol as in
+// build_nd.
+int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initial, const double* thr_mol = nullptr) {
   int r;
-  HIPOK(hipEventRecord(c->ev0, c->stream));
+  if (!c->capturing) HIPOK(hipEventRecord(c->ev0, c->stream));
   if ((r = zero_build_state(c))) return r;
   if ((r = launch_maxdm(c, dP))) return r;
   for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
@@ -723,18 +849,21 @@ int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initia
     s.pairs = c->d_pairs;
     s.schw = c->d_pair_schw[t];
     s.maxdm = c->d_maxdm;
+    s.md_stride = (long long)c->nsp * c->nsp;
+    s.thr_mol = thr_mol;
     s.list = c->d_diag_list[t];
     s.count = c->d_counts + cls;
     s.stats = c->d_stats + 3 * cls;
     s.thr = thr;
     s.first = c->offset[t];
     s.npair = n;
-    s.nsh = c->nsh;
+    s.nsh = c->nsp;
     s.check = initial ? 0 : 1;
     s.all_prims = kClassKernels[cls].all_prims;
     hipLaunchKernelGGL(k_screen_diag, dim3(grid_for(n, 256, 1024)), dim3(256), 0, c->stream, s);
     HIPOK(hipGetLastError());
     ClassArgs a = base_args(c, dP, thr);
+    a.thr_mol = thr_mol;
     a.list = c->d_diag_list[t];
     a.count = c->d_counts + cls;
     a.diag = 1;
@@ -756,22 +885,90 @@ int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initia
   }
   hipLaunchKernelGGL(k_pairbuf_to_x, dim3(grid_for((long long)c->totpairs * HFX_PAIRBUF_STRIDE, 256, 2048)), dim3(256), 0,
                      c->stream, c->d_pairbuf, c->d_pairs, c->d_shells, c->totpairs, c->d_X, c->nb);
-  hipLaunchKernelGGL(k_add_sym, dim3(grid_for((long long)c->nb * c->nb, 256, 2048)), dim3(256), 0, c->stream, c->d_X, dF, c->nb);
+  hipLaunchKernelGGL(k_add_sym, dim3(grid_for((long long)c->nb * c->nb * c->nmol, 256, 4096)), dim3(256), 0, c->stream,
+                     c->d_X, dF, c->nb, c->nmol);
   HIPOK(hipGetLastError());
-  HIPOK(hipEventRecord(c->ev1, c->stream));
+  if (!c->capturing) HIPOK(hipEventRecord(c->ev1, c->stream));
   c->last_kind = 2;
+  return HFX_OK;
+}
+
+// diag + non-diag build of every molecule with per-molecule thresholds
+// (device arrays), F optionally cleared first: one SCF iteration's build.
+int build_full(hfx_ctx* c, const GraphKey& k) {
+  int r;
+  if (k.zero_f) {
+    const size_t n = (size_t)c->nb * c->nb * c->nmol;
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)n, 256, 4096)), dim3(256), 0, c->stream, k.F, n, 0.0);
+    HIPOK(hipGetLastError());
+  }
+  if ((r = build_diag(c, k.P, k.F, 0.0, false, k.thr_diag))) return r;
+  return build_nd(c, k.P, k.F, 0.0, k.mode, k.split, k.thr_nd, k.split_mol);
+}
+
+// The same build as a HIP graph: replayed when one with these arguments was
+// captured before; otherwise run once in the fixed launch sequence of
+// device-sized chunks (which sizes every buffer it needs), then captured
+// from the context's stream for the next call.
+int graph_build(hfx_ctx* c, const GraphKey& k) {
+  if (c->graphs_off) return build_full(c, k);
+  for (const CachedGraph& g : c->graphs)
+    if (g.key == k) {
+      HIPOK(hipGraphLaunch(g.exec, c->stream));
+      return HFX_OK;
+    }
+  const bool prev = c->device_chunks;
+  c->device_chunks = true;
+  int r = build_full(c, k);
+  if (r) {
+    c->device_chunks = prev;
+    return r;
+  }
+  CachedGraph g;
+  g.key = k;
+  // relaxed: other threads of the process (other contexts' jobs) keep
+  // allocating and copying while this stream is captured
+  hipError_t e;
+  {
+    hfx::CaptureSection cs;
+    e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+    if (e == hipSuccess) {
+      c->capturing = true;
+      r = build_full(c, k);
+      c->capturing = false;
+      const hipError_t e2 = hipStreamEndCapture(c->stream, &g.graph);
+      if (e2 != hipSuccess) e = e2;
+    }
+  }
+  if (r == HFX_OK && e == hipSuccess) {
+    hfx::DeviceSection ds;
+    e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+  }
+  c->device_chunks = prev;
+  if (r != HFX_OK || e != hipSuccess) {
+    // nothing captured ran; F holds the uncaptured build made above
+    hfx::DeviceSection ds;
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    if (g.graph) hipGraphDestroy(g.graph);
+    (void)hipGetLastError();
+    fprintf(stderr, "hfx: build not captured (%s); this context's builds run uncaptured\n",
+            r != HFX_OK ? "build error" : hipGetErrorString(e));
+    c->graphs_off = true;
+    return HFX_OK;
+  }
+  c->graphs.push_back(g);
   return HFX_OK;
 }
 
 // Descending Schwarz order and truncation index, as index_sort / find_threshold
 // (index_sort.f90:1-119) define them: `last` is the position of the first
 // pair below shthresh (that pair included), or n when none is below.
-void sort_schwarz(hfx_ctx* c, int t) {
-  const int n = c->npair[t];
-  std::vector<int>& o = c->order[t];
+void sort_schwarz(MolPart& mp, int t) {
+  const int n = mp.npair[t];
+  std::vector<int>& o = mp.order[t];
   o.resize(n);
   for (int i = 0; i < n; ++i) o[i] = i;
-  const std::vector<double>& s = c->schw[t];
+  const std::vector<double>& s = mp.schw[t];
   std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return s[a] > s[b]; });
   int last = n;
   if (n > 0 && s[o[0]] < kShthresh) {
@@ -782,18 +979,18 @@ void sort_schwarz(hfx_ctx* c, int t) {
     for (int i = 0; i < n; ++i)
       if (s[o[i]] < kShthresh) { last = i + 1; break; }
   }
-  c->last[t] = last;
+  mp.last[t] = last;
 }
 
 int copy_fock_host(hfx_ctx* c, const double* P, double* F, int nb) {
-  const size_t bytes = (size_t)nb * nb * sizeof(double);
+  const size_t bytes = (size_t)nb * nb * c->nmol * sizeof(double);
   HIPOK(hipMemcpyAsync(c->d_P, P, bytes, hipMemcpyHostToDevice, c->stream));
   HIPOK(hipMemcpyAsync(c->d_F, F, bytes, hipMemcpyHostToDevice, c->stream));
   return HFX_OK;
 }
 
 int fetch_fock_host(hfx_ctx* c, double* F, int nb) {
-  const size_t bytes = (size_t)nb * nb * sizeof(double);
+  const size_t bytes = (size_t)nb * nb * c->nmol * sizeof(double);
   HIPOK(hipMemcpyAsync(F, c->d_F, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPOK(hipStreamSynchronize(c->stream));
   return HFX_OK;
@@ -805,11 +1002,12 @@ int finish_schwarz(hfx_ctx* c) {
   int r;
   if (!c->have_schwarz) {
     std::vector<double> s(c->totpairs);
-    HIPOK(hipMemcpy(s.data(), c->d_schw_out, s.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
-      for (int k = 0; k < c->npair[t]; ++k) c->schw[t][k] = s[c->offset[t] + k];
-      sort_schwarz(c, t);
-    }
+    HIPOK(copy_sync(c->stream, s.data(), c->d_schw_out, s.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (MolPart& mp : c->mols)
+      for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
+        for (int k = 0; k < mp.npair[t]; ++k) mp.schw[t][k] = s[mp.pbeg[t] + k];
+        sort_schwarz(mp, t);
+      }
     if ((r = upload_active(c))) return r;
   }
   return HFX_OK;
@@ -821,112 +1019,139 @@ extern "C" {
 
 const char* hfx_version(void) { return "hfx-mi355x 0.1 (gfx950)"; }
 
-int hfx_ctx_create(const hfx_mol_desc* d, hfx_ctx** out) {
-  if (!d || !out || d->nb <= 0 || d->nsh <= 0 || !d->shtyp || !d->nzet || !d->aosh || !d->xsh || !d->zeta ||
-      !d->djk || d->maxprim <= 0 || d->maxprim > 16)
-    return HFX_E_ARG;
-  *out = nullptr;
-  hfx_ctx* c = new hfx_ctx();
-  c->nb = d->nb;
-  c->nsh = d->nsh;
-  bool hasP = false, hasD = false, hasL = false;
-  c->shells.resize(d->nsh);
-  for (int i = 0; i < d->nsh; ++i) {
-    Shell& s = c->shells[i];
-    s.type = d->shtyp[i];
-    if (letter(s.type) < 0) { delete c; return HFX_E_ARG; }
-    hasP |= s.type == 2; hasD |= s.type == 3; hasL |= s.type == 12;
-    s.nprim = d->nzet[i];
-    if (s.nprim <= 0 || s.nprim > d->maxprim || s.nprim > kMaxShellPrims) { delete c; return HFX_E_UNSUPPORTED; }
-    s.ao = d->aosh[i] - 1;
-    if (s.ao < 0 || s.ao + ncomp_of(s.type) > d->nb) { delete c; return HFX_E_ARG; }
-    for (int k = 0; k < 3; ++k) s.x[k] = d->xsh[3 * i + k];
-    for (int k = 0; k < s.nprim; ++k) {
-      s.zeta[k] = d->zeta[(size_t)d->maxprim * i + k];
-      s.djk[k] = d->djk[(size_t)d->maxprim * i + k];
-      s.djkl[k] = (s.type == 12 && d->djkl) ? d->djkl[(size_t)d->maxprim * i + k] : 1.0;
-    }
-  }
-  if (hasP && hasD && hasL) { delete c; return HFX_E_UNSUPPORTED; }  // spread_basis.f90:174-177
+int hfx_ctx_create(const hfx_mol_desc* d, hfx_ctx** out) { return hfx_batch_create(d, 1, out); }
 
-  // pair lists
+int hfx_batch_create(const hfx_mol_desc* descs, int nmol, hfx_ctx** out) {
+  if (!descs || !out || nmol < 1) return HFX_E_ARG;
+  *out = nullptr;
+  for (int m = 0; m < nmol; ++m) {
+    const hfx_mol_desc* d = descs + m;
+    if (d->nb <= 0 || d->nsh <= 0 || !d->shtyp || !d->nzet || !d->aosh || !d->xsh || !d->zeta || !d->djk ||
+        d->maxprim <= 0 || d->maxprim > 16)
+      return HFX_E_ARG;
+  }
+  hfx_ctx* c = new hfx_ctx();
+  c->nmol = nmol;
+  c->mols.resize(nmol);
+  for (int m = 0; m < nmol; ++m) {
+    const hfx_mol_desc* d = descs + m;
+    MolPart& mp = c->mols[m];
+    mp.nb = d->nb;
+    mp.nsh = d->nsh;
+    mp.sh0 = c->nsh;
+    c->nb = std::max(c->nb, d->nb);
+    c->nsp = std::max(c->nsp, d->nsh);
+    c->nsh += d->nsh;
+    bool hasP = false, hasD = false, hasL = false;
+    for (int i = 0; i < d->nsh; ++i) {
+      Shell s;
+      s.type = d->shtyp[i];
+      if (letter(s.type) < 0) { delete c; return HFX_E_ARG; }
+      hasP |= s.type == 2; hasD |= s.type == 3; hasL |= s.type == 12;
+      s.nprim = d->nzet[i];
+      if (s.nprim <= 0 || s.nprim > d->maxprim || s.nprim > kMaxShellPrims) { delete c; return HFX_E_UNSUPPORTED; }
+      s.ao = d->aosh[i] - 1;
+      if (s.ao < 0 || s.ao + ncomp_of(s.type) > d->nb) { delete c; return HFX_E_ARG; }
+      for (int k = 0; k < 3; ++k) s.x[k] = d->xsh[3 * i + k];
+      for (int k = 0; k < s.nprim; ++k) {
+        s.zeta[k] = d->zeta[(size_t)d->maxprim * i + k];
+        s.djk[k] = d->djk[(size_t)d->maxprim * i + k];
+        s.djkl[k] = (s.type == 12 && d->djkl) ? d->djkl[(size_t)d->maxprim * i + k] : 1.0;
+      }
+      c->shells.push_back(s);
+    }
+    if (hasP && hasD && hasL) { delete c; return HFX_E_UNSUPPORTED; }  // spread_basis.f90:174-177
+  }
+
+  // pair lists: type-major, then molecule by molecule in each molecule's list order
   const double rt2pi54 = std::sqrt(2.0 * M_PI * M_PI * std::sqrt(M_PI));  // nrtype.f90:12-13
   bool schwarz_given = true;
   int tot = 0;
+  std::vector<int> gprs;   // 2 global shells per global pair
+  std::vector<int> gmol;   // molecule per global pair
   for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
-    const int n = d->npair[t];
-    c->npair[t] = n;
     c->offset[t] = tot;
-    tot += n;
-    if (n == 0) continue;
-    if (!d->prs[t]) { delete c; return HFX_E_ARG; }
-    c->prs[t].resize(2 * n);
-    for (int k = 0; k < n; ++k) {
-      const int s1 = d->prs[t][2 * k] - 1, s2 = d->prs[t][2 * k + 1] - 1;
-      if (s1 < 0 || s2 < 0 || s1 >= d->nsh || s2 >= d->nsh) { delete c; return HFX_E_ARG; }
-      if (kLetters[letter(c->shells[s1].type)] != kPairNames[t][0] ||
-          kLetters[letter(c->shells[s2].type)] != kPairNames[t][1]) { delete c; return HFX_E_ARG; }
-      c->prs[t][2 * k] = s1;
-      c->prs[t][2 * k + 1] = s2;
-    }
-    if (d->schw[t] && d->order[t]) {
-      c->schw[t].assign(d->schw[t], d->schw[t] + n);
-      c->order[t].resize(n);
+    for (int m = 0; m < nmol; ++m) {
+      const hfx_mol_desc* d = descs + m;
+      MolPart& mp = c->mols[m];
+      const int n = d->npair[t];
+      mp.npair[t] = n;
+      mp.pbeg[t] = tot;
+      tot += n;
+      if (n == 0) continue;
+      if (!d->prs[t] || n < 0) { delete c; return HFX_E_ARG; }
       for (int k = 0; k < n; ++k) {
-        c->order[t][k] = d->order[t][k] - 1;
-        if (c->order[t][k] < 0 || c->order[t][k] >= n) { delete c; return HFX_E_ARG; }
+        const int s1 = d->prs[t][2 * k] - 1, s2 = d->prs[t][2 * k + 1] - 1;
+        if (s1 < 0 || s2 < 0 || s1 >= d->nsh || s2 >= d->nsh) { delete c; return HFX_E_ARG; }
+        if (kLetters[letter(c->shells[mp.sh0 + s1].type)] != kPairNames[t][0] ||
+            kLetters[letter(c->shells[mp.sh0 + s2].type)] != kPairNames[t][1]) { delete c; return HFX_E_ARG; }
+        gprs.push_back(mp.sh0 + s1);
+        gprs.push_back(mp.sh0 + s2);
+        gmol.push_back(m);
       }
-      c->last[t] = d->last[t];
-      if (c->last[t] < 0 || c->last[t] > n) { delete c; return HFX_E_ARG; }
-    } else {
-      schwarz_given = false;
-      c->schw[t].assign(n, 0.0);
-      c->order[t].resize(n);
-      for (int k = 0; k < n; ++k) c->order[t][k] = k;
-      c->last[t] = 0;
+      if (d->schw[t] && d->order[t]) {
+        mp.schw[t].assign(d->schw[t], d->schw[t] + n);
+        mp.order[t].resize(n);
+        for (int k = 0; k < n; ++k) {
+          mp.order[t][k] = d->order[t][k] - 1;
+          if (mp.order[t][k] < 0 || mp.order[t][k] >= n) { delete c; return HFX_E_ARG; }
+        }
+        mp.last[t] = d->last[t];
+        if (mp.last[t] < 0 || mp.last[t] > n) { delete c; return HFX_E_ARG; }
+      } else {
+        schwarz_given = false;
+        mp.schw[t].assign(n, 0.0);
+        mp.order[t].resize(n);
+        for (int k = 0; k < n; ++k) mp.order[t][k] = k;
+        mp.last[t] = 0;
+      }
     }
+    c->npair[t] = tot - c->offset[t];
   }
   c->totpairs = tot;
+  const size_t nb2 = (size_t)c->nb * c->nb;
   // primitive pairs, precalc_kabs.f90:94-115, with the negab ones moved last
   c->pairs.resize(tot);
-  for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
-    for (int k = 0; k < c->npair[t]; ++k) {
-      const Shell& A = c->shells[c->prs[t][2 * k]];
-      const Shell& B = c->shells[c->prs[t][2 * k + 1]];
-      PairRec& pr = c->pairs[c->offset[t] + k];
-      pr.ax = A.x[0]; pr.ay = A.x[1]; pr.az = A.x[2];
-      pr.bx = B.x[0]; pr.by = B.x[1]; pr.bz = B.x[2];
-      pr.sha = c->prs[t][2 * k];
-      pr.shb = c->prs[t][2 * k + 1];
-      pr.aoa = A.ao;
-      pr.aob = B.ao;
-      pr.rank = -1;
-      const double dx = A.x[0] - B.x[0], dy = A.x[1] - B.x[1], dz = A.x[2] - B.x[2];
-      const double rab2 = dx * dx + dy * dy + dz * dz;
-      std::vector<PrimRec> keep, neg;
-      for (int za = 0; za < A.nprim; ++za) {
-        for (int zb = 0; zb < B.nprim; ++zb) {
-          PrimRec r;
-          const double a = A.zeta[za], b = B.zeta[zb];
-          r.p = a + b;
-          r.pinv = 1.0 / r.p;
-          const double q = a * b * r.pinv;
-          r.Px = (a * A.x[0] + b * B.x[0]) * r.pinv;
-          r.Py = (a * A.x[1] + b * B.x[1]) * r.pinv;
-          r.Pz = (a * A.x[2] + b * B.x[2]) * r.pinv;
-          const double dd = A.djk[za] * B.djk[zb];
-          r.K = rt2pi54 * std::exp(-q * rab2) * r.pinv * dd;
-          r.wa = A.type == 12 ? A.djkl[za] : 1.0;
-          r.wb = B.type == 12 ? B.djkl[zb] : 1.0;
-          if (std::fabs(r.K) * std::sqrt(r.pinv) < kNegab) neg.push_back(r); else keep.push_back(r);
-        }
+  for (int g = 0; g < tot; ++g) {
+    const int m = gmol[g];
+    const Shell& A = c->shells[gprs[2 * g]];
+    const Shell& B = c->shells[gprs[2 * g + 1]];
+    PairRec& pr = c->pairs[g];
+    pr.ax = A.x[0]; pr.ay = A.x[1]; pr.az = A.x[2];
+    pr.bx = B.x[0]; pr.by = B.x[1]; pr.bz = B.x[2];
+    pr.sha = gprs[2 * g];
+    pr.shb = gprs[2 * g + 1];
+    pr.aoa = A.ao;
+    pr.aob = B.ao;
+    pr.mol = m;
+    pr.lsa = pr.sha - c->mols[m].sh0;
+    pr.lsb = pr.shb - c->mols[m].sh0;
+    pr.mbase = (long long)m * (long long)nb2;
+    const double dx = A.x[0] - B.x[0], dy = A.x[1] - B.x[1], dz = A.x[2] - B.x[2];
+    const double rab2 = dx * dx + dy * dy + dz * dz;
+    std::vector<PrimRec> keep, neg;
+    for (int za = 0; za < A.nprim; ++za) {
+      for (int zb = 0; zb < B.nprim; ++zb) {
+        PrimRec r;
+        const double a = A.zeta[za], b = B.zeta[zb];
+        r.p = a + b;
+        r.pinv = 1.0 / r.p;
+        const double q = a * b * r.pinv;
+        r.Px = (a * A.x[0] + b * B.x[0]) * r.pinv;
+        r.Py = (a * A.x[1] + b * B.x[1]) * r.pinv;
+        r.Pz = (a * A.x[2] + b * B.x[2]) * r.pinv;
+        const double dd = A.djk[za] * B.djk[zb];
+        r.K = rt2pi54 * std::exp(-q * rab2) * r.pinv * dd;
+        r.wa = A.type == 12 ? A.djkl[za] : 1.0;
+        r.wb = B.type == 12 ? B.djkl[zb] : 1.0;
+        if (std::fabs(r.K) * std::sqrt(r.pinv) < kNegab) neg.push_back(r); else keep.push_back(r);
       }
-      pr.p0 = (int)c->prims.size();
-      pr.np = (int)keep.size();
-      pr.npall = (int)(keep.size() + neg.size());
-      c->prims.insert(c->prims.end(), keep.begin(), keep.end());
-      c->prims.insert(c->prims.end(), neg.begin(), neg.end());
     }
+    pr.p0 = (int)c->prims.size();
+    pr.np = (int)keep.size();
+    pr.npall = (int)(keep.size() + neg.size());
+    c->prims.insert(c->prims.end(), keep.begin(), keep.end());
+    c->prims.insert(c->prims.end(), neg.begin(), neg.end());
   }
   // classes
   for (int cl = 0; cl < HFX_NCLASSES; ++cl) {
@@ -951,36 +1176,41 @@ int hfx_ctx_create(const hfx_mol_desc* d, hfx_ctx** out) {
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(HFX_E_HIP);
   std::vector<ShellRec> sr(c->nsh);
   for (int i = 0; i < c->nsh; ++i) sr[i] = ShellRec{c->shells[i].type, c->shells[i].ao, ncomp_of(c->shells[i].type), 0};
+  std::vector<int2> molsh(nmol);
+  for (int m = 0; m < nmol; ++m) molsh[m] = make_int2(c->mols[m].sh0, c->mols[m].nsh);
   std::vector<double> b64;
   std::vector<float> b32;
   boys_table(b64, b32);
-  const size_t nb2 = (size_t)c->nb * c->nb;
+  const size_t nmat = nb2 * nmol;
   if ((r = dalloc(&c->d_pairs, c->pairs.size())) || (r = dalloc(&c->d_prims, c->prims.size())) ||
-      (r = dalloc(&c->d_shells, sr.size())) ||
+      (r = dalloc(&c->d_shells, sr.size())) || (r = dalloc(&c->d_molsh, molsh.size())) ||
       (r = dalloc(&c->d_pairbuf, (size_t)c->totpairs * HFX_PAIRBUF_STRIDE)) || (r = dalloc(&c->d_boys64, b64.size())) ||
-      (r = dalloc(&c->d_boys32, b32.size())) || (r = dalloc(&c->d_P, nb2)) || (r = dalloc(&c->d_F, nb2)) ||
-      (r = dalloc(&c->d_X, nb2)) || (r = dalloc(&c->d_maxdm, (size_t)c->nsh * c->nsh)) || (r = dalloc(&c->d_pmax, 1)) ||
-      (r = dalloc(&c->d_schw_out, (size_t)tot)) || (r = dalloc(&c->d_counts, 2 * HFX_NCLASSES)) ||
-      (r = dalloc(&c->d_stats, 2 * HFX_NCLASSES * 3)))
+      (r = dalloc(&c->d_boys32, b32.size())) || (r = dalloc(&c->d_P, nmat)) || (r = dalloc(&c->d_F, nmat)) ||
+      (r = dalloc(&c->d_X, nmat)) || (r = dalloc(&c->d_maxdm, (size_t)c->nsp * c->nsp * nmol)) ||
+      (r = dalloc(&c->d_pmax, nmol)) || (r = dalloc(&c->d_schw_out, (size_t)tot)) ||
+      (r = dalloc(&c->d_counts, 2 * HFX_NCLASSES)) || (r = dalloc(&c->d_stats, 2 * HFX_NCLASSES * 3)))
     return fail(r);
   for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
     if (c->npair[t] == 0) continue;
     if ((r = dalloc(&c->d_diag_list[t], c->npair[t])) || (r = dalloc(&c->d_pair_schw[t], c->npair[t])))
       return fail(r);
   }
-  if (hipMemcpy(c->d_prims, c->prims.data(), c->prims.size() * sizeof(PrimRec), hipMemcpyHostToDevice) ||
-      hipMemcpy(c->d_shells, sr.data(), sr.size() * sizeof(ShellRec), hipMemcpyHostToDevice) ||
-      hipMemcpy(c->d_boys64, b64.data(), b64.size() * sizeof(double), hipMemcpyHostToDevice) ||
-      hipMemcpy(c->d_boys32, b32.data(), b32.size() * sizeof(float), hipMemcpyHostToDevice) ||
-      hipMemcpy(c->d_pairs, c->pairs.data(), c->pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice))
+  if (copy_sync(c->stream, c->d_prims, c->prims.data(), c->prims.size() * sizeof(PrimRec), hipMemcpyHostToDevice) ||
+      copy_sync(c->stream, c->d_shells, sr.data(), sr.size() * sizeof(ShellRec), hipMemcpyHostToDevice) ||
+      copy_sync(c->stream, c->d_molsh, molsh.data(), molsh.size() * sizeof(int2), hipMemcpyHostToDevice) ||
+      copy_sync(c->stream, c->d_boys64, b64.data(), b64.size() * sizeof(double), hipMemcpyHostToDevice) ||
+      copy_sync(c->stream, c->d_boys32, b32.data(), b32.size() * sizeof(float), hipMemcpyHostToDevice) ||
+      copy_sync(c->stream, c->d_pairs, c->pairs.data(), c->pairs.size() * sizeof(PairRec), hipMemcpyHostToDevice))
     return fail(HFX_E_HIP);
   if (schwarz_given) {
     if ((r = upload_active(c))) return fail(r);
   } else {
-    for (int t = 0; t < HFX_NPAIRTYPES; ++t)
-      if (c->npair[t] && hipMemcpy(c->d_pair_schw[t], c->schw[t].data(), c->npair[t] * sizeof(double),
-                                   hipMemcpyHostToDevice))
+    for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
+      if (!c->npair[t]) continue;
+      std::vector<double> ps(c->npair[t], 0.0);
+      if (copy_sync(c->stream, c->d_pair_schw[t], ps.data(), ps.size() * sizeof(double), hipMemcpyHostToDevice))
         return fail(HFX_E_HIP);
+    }
   }
   c->counted = true;
   live_contexts.fetch_add(1);
@@ -991,13 +1221,15 @@ int hfx_ctx_create(const hfx_mol_desc* d, hfx_ctx** out) {
 void hfx_ctx_destroy(hfx_ctx* c) {
   if (!c) return;
   sync_all_streams(c);
+  drop_graphs(c);
   for (auto& cp : c->classes) {
     dfree(cp.list64); dfree(cp.list32); dfree(cp.blocks64); dfree(cp.blocks32);
   }
   for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
     dfree(c->d_act[t]); dfree(c->d_act_schw[t]); dfree(c->d_act_info[t]); dfree(c->d_pair_schw[t]);
-    dfree(c->d_diag_list[t]);
+    dfree(c->d_diag_list[t]); dfree(c->d_bmol[t]); dfree(c->d_aseg[t]);
   }
+  dfree(c->d_molsh);
   dfree(c->d_pairs); dfree(c->d_prims); dfree(c->d_shells); dfree(c->d_pairbuf); dfree(c->d_boys64); dfree(c->d_boys32);
   dfree(c->d_P); dfree(c->d_P32); dfree(c->d_F); dfree(c->d_X); dfree(c->d_maxdm); dfree(c->d_schw_out);
   dfree(c->d_onee_items); dfree(c->d_sh_type); dfree(c->d_sh_np); dfree(c->d_sh_ao);
@@ -1068,27 +1300,39 @@ int hfx_fock_diag(hfx_ctx* c, const double* P, double* F, int nb, double thr) {
   return fetch_fock_host(c, F, nb);
 }
 
-// S, T, V on the device: work items per (la, lb) of the shell-pair parts,
-// one generated kernel pair per (la, lb); copies of the three matrices back.
-int hfx_ctx_onee(hfx_ctx* c, int natom, const double* charges, const double* coords, double* S, double* T,
-                 double* V) {
-  if (!c || !S || !T || !V || natom < 0 || (natom > 0 && (!charges || !coords))) return HFX_E_ARG;
+// S, T, V on the device: work items per (molecule, la, lb) of the shell-pair
+// parts, one generated kernel pair per (la, lb); each molecule's matrices in
+// its block of the stacked outputs, its own nuclei for V.  natom[m] atoms of
+// molecule m, their charges and coordinates concatenated.
+int hfx_batch_onee(hfx_ctx* c, const int* natom, const double* charges, const double* coords, double* S, double* T,
+                   double* V) {
+  if (!c || !natom || !S || !T || !V) return HFX_E_ARG;
+  int ntot = 0;
+  for (int m = 0; m < c->nmol; ++m) {
+    if (natom[m] < 0) return HFX_E_ARG;
+    ntot += natom[m];
+  }
+  if (ntot > 0 && (!charges || !coords)) return HFX_E_ARG;
   const int nsh = c->nsh, nb = c->nb;
   if (!c->d_onee_items) {
     auto nparts = [](int t) { return t == 12 ? 2 : 1; };
     auto part_l = [](int t, int p) { return t == 1 ? 0 : t == 2 ? 1 : t == 3 ? 2 : p; };
-    std::vector<int4> items[3][3];
-    for (int i = 0; i < nsh; ++i)
-      for (int j = 0; j <= i; ++j)
-        for (int pa = 0; pa < nparts(c->shells[i].type); ++pa)
-          for (int pb = 0; pb < nparts(c->shells[j].type); ++pb)
-            items[part_l(c->shells[i].type, pa)][part_l(c->shells[j].type, pb)].push_back(make_int4(i, j, pa, pb));
     std::vector<int4> all;
-    for (int la = 0; la < 3; ++la)
-      for (int lb = 0; lb < 3; ++lb) {
-        c->onee_count[la][lb] = (int)items[la][lb].size();
-        all.insert(all.end(), items[la][lb].begin(), items[la][lb].end());
-      }
+    c->onee_count.assign((size_t)c->nmol * 9, 0);
+    for (int m = 0; m < c->nmol; ++m) {
+      const MolPart& mp = c->mols[m];
+      std::vector<int4> items[3][3];
+      for (int i = mp.sh0; i < mp.sh0 + mp.nsh; ++i)
+        for (int j = mp.sh0; j <= i; ++j)
+          for (int pa = 0; pa < nparts(c->shells[i].type); ++pa)
+            for (int pb = 0; pb < nparts(c->shells[j].type); ++pb)
+              items[part_l(c->shells[i].type, pa)][part_l(c->shells[j].type, pb)].push_back(make_int4(i, j, pa, pb));
+      for (int la = 0; la < 3; ++la)
+        for (int lb = 0; lb < 3; ++lb) {
+          c->onee_count[(size_t)m * 9 + la * 3 + lb] = (int)items[la][lb].size();
+          all.insert(all.end(), items[la][lb].begin(), items[la][lb].end());
+        }
+    }
     const int mp = 16;
     std::vector<int> ty(nsh), np(nsh), ao(nsh);
     std::vector<double> xyz(3 * nsh), z((size_t)mp * nsh), cc((size_t)mp * nsh), cl((size_t)mp * nsh);
@@ -1108,57 +1352,73 @@ int hfx_ctx_onee(hfx_ctx* c, int natom, const double* charges, const double* coo
         dalloc(&c->d_sh_ao, nsh) || dalloc(&c->d_sh_xyz, xyz.size()) || dalloc(&c->d_sh_z, z.size()) ||
         dalloc(&c->d_sh_c, cc.size()) || dalloc(&c->d_sh_cl, cl.size()))
       return HFX_E_NOMEM;
-    HIPOK(hipMemcpy(c->d_onee_items, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(c->d_sh_type, ty.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(c->d_sh_np, np.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(c->d_sh_ao, ao.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(c->d_sh_xyz, xyz.data(), xyz.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(c->d_sh_z, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(c->d_sh_c, cc.data(), cc.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(c->d_sh_cl, cl.data(), cl.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPOK(copy_sync(c->stream, c->d_onee_items, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPOK(copy_sync(c->stream, c->d_sh_type, ty.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
+    HIPOK(copy_sync(c->stream, c->d_sh_np, np.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
+    HIPOK(copy_sync(c->stream, c->d_sh_ao, ao.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
+    HIPOK(copy_sync(c->stream, c->d_sh_xyz, xyz.data(), xyz.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPOK(copy_sync(c->stream, c->d_sh_z, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPOK(copy_sync(c->stream, c->d_sh_c, cc.data(), cc.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPOK(copy_sync(c->stream, c->d_sh_cl, cl.data(), cl.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   double *d_q = nullptr, *d_xyz = nullptr, *d_S = nullptr, *d_T = nullptr, *d_V = nullptr;
-  const size_t nb2 = (size_t)nb * nb;
+  const size_t nmat = (size_t)nb * nb * c->nmol;
   int r = HFX_OK;
-  if (dalloc(&d_q, std::max(natom, 1)) || dalloc(&d_xyz, 3 * std::max(natom, 1)) || dalloc(&d_S, nb2) ||
-      dalloc(&d_T, nb2) || dalloc(&d_V, nb2))
+  if (dalloc(&d_q, std::max(ntot, 1)) || dalloc(&d_xyz, 3 * std::max(ntot, 1)) || dalloc(&d_S, nmat) ||
+      dalloc(&d_T, nmat) || dalloc(&d_V, nmat))
     r = HFX_E_NOMEM;
-  if (r == HFX_OK && natom > 0) {
-    if (hipMemcpy(d_q, charges, natom * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d_xyz, coords, 3 * natom * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+  if (r == HFX_OK && ntot > 0) {
+    if (copy_sync(c->stream, d_q, charges, ntot * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        copy_sync(c->stream, d_xyz, coords, 3 * ntot * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       r = HFX_E_HIP;
+  }
+  if (r == HFX_OK) {   // the padding of smaller molecules' blocks stays zero
+    for (double* x : {d_S, d_T, d_V})
+      hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)nmat, 256, 4096)), dim3(256), 0, c->stream, x, nmat, 0.0);
   }
   if (r == HFX_OK) {
     OneeArgs a;
     memset(&a, 0, sizeof(a));
     a.sh_type = c->d_sh_type; a.sh_np = c->d_sh_np; a.sh_ao = c->d_sh_ao; a.sh_xyz = c->d_sh_xyz;
     a.sh_z = c->d_sh_z; a.sh_c = c->d_sh_c; a.sh_cl = c->d_sh_cl; a.maxprim = 16;
-    a.charges = d_q; a.coords = d_xyz; a.natom = natom; a.boys64 = c->d_boys64;
-    a.S = d_S; a.T = d_T; a.V = d_V; a.nb = nb;
-    int off = 0;
-    for (int la = 0; la < 3 && r == HFX_OK; ++la)
-      for (int lb = 0; lb < 3 && r == HFX_OK; ++lb) {
-        const int n = c->onee_count[la][lb];
-        if (n == 0) continue;
-        a.items = c->d_onee_items + off;
-        a.nitems = n;
-        off += n;
-        const int grid = grid_for(n, HFX_NT, c->ncu * 8);
-        const size_t shm = (size_t)HFX_BOYS_ROWS * (la + lb + 8) * sizeof(double);
-        hipLaunchKernelGGL(kOneeST[la][lb], dim3(grid), dim3(HFX_NT), 0, c->stream, a);
-        hipLaunchKernelGGL(kOneeV[la][lb], dim3(grid), dim3(HFX_NT), shm, c->stream, a);
-        if (hipGetLastError() != hipSuccess) r = HFX_E_HIP;
-      }
+    a.boys64 = c->d_boys64;
+    a.nb = nb;
+    int off = 0, atom0 = 0;
+    for (int m = 0; m < c->nmol && r == HFX_OK; ++m) {
+      const size_t base = (size_t)m * nb * nb;
+      a.charges = d_q + atom0; a.coords = d_xyz + 3 * atom0; a.natom = natom[m];
+      a.S = d_S + base; a.T = d_T + base; a.V = d_V + base;
+      atom0 += natom[m];
+      for (int la = 0; la < 3 && r == HFX_OK; ++la)
+        for (int lb = 0; lb < 3 && r == HFX_OK; ++lb) {
+          const int n = c->onee_count[(size_t)m * 9 + la * 3 + lb];
+          if (n == 0) continue;
+          a.items = c->d_onee_items + off;
+          a.nitems = n;
+          off += n;
+          const int grid = grid_for(n, HFX_NT, c->ncu * 8);
+          const size_t shm = (size_t)HFX_BOYS_ROWS * (la + lb + 8) * sizeof(double);
+          hipLaunchKernelGGL(kOneeST[la][lb], dim3(grid), dim3(HFX_NT), 0, c->stream, a);
+          hipLaunchKernelGGL(kOneeV[la][lb], dim3(grid), dim3(HFX_NT), shm, c->stream, a);
+          if (hipGetLastError() != hipSuccess) r = HFX_E_HIP;
+        }
+    }
   }
   if (r == HFX_OK &&
-      (hipMemcpyAsync(S, d_S, nb2 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-       hipMemcpyAsync(T, d_T, nb2 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-       hipMemcpyAsync(V, d_V, nb2 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      (hipMemcpyAsync(S, d_S, nmat * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+       hipMemcpyAsync(T, d_T, nmat * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+       hipMemcpyAsync(V, d_V, nmat * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
        hipStreamSynchronize(c->stream) != hipSuccess))
     r = HFX_E_HIP;
   hipStreamSynchronize(c->stream);   // nothing queued may still use the temporaries
   dfree(d_q); dfree(d_xyz); dfree(d_S); dfree(d_T); dfree(d_V);
   return r;
+}
+
+int hfx_ctx_onee(hfx_ctx* c, int natom, const double* charges, const double* coords, double* S, double* T,
+                 double* V) {
+  if (!c || natom < 0 || c->nmol != 1) return HFX_E_ARG;
+  return hfx_batch_onee(c, &natom, charges, coords, S, T, V);
 }
 
 int hfx_fock_initial_diag(hfx_ctx* c, const double* P, double* F, int nb, double thr) {
@@ -1194,6 +1454,8 @@ void* hfx_ctx_stream(hfx_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 int hfx_ctx_nb(hfx_ctx* c) { return c ? c->nb : HFX_E_ARG; }
 
+int hfx_ctx_nmol(hfx_ctx* c) { return c ? c->nmol : HFX_E_ARG; }
+
 int hfx_fock_nd_device(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, void* stream) {
   return c ? hfx::fock_nd_device_split(c, dP, dF, thr, mode, stream, c->split) : HFX_E_ARG;
 }
@@ -1218,6 +1480,55 @@ int hfx::fock_nd_device_split(hfx_ctx* c, const double* dP, double* dF, double t
 }
 
 double hfx::ctx_split(const hfx_ctx* c) { return c ? c->split : 0.0; }
+
+int hfx::ctx_dims(const hfx_ctx* c, int* nmol, int* nb_stride, std::vector<int>* nbs) {
+  if (!c) return HFX_E_ARG;
+  if (nmol) *nmol = c->nmol;
+  if (nb_stride) *nb_stride = c->nb;
+  if (nbs) {
+    nbs->clear();
+    for (const MolPart& mp : c->mols) nbs->push_back(mp.nb);
+  }
+  return HFX_OK;
+}
+
+extern "C" int hfx_batch_fock_device(hfx_ctx* c, const double* dP, double* dF, const double* d_thr_diag,
+                                     const double* d_thr_nd, const double* d_split, int mode, int flags,
+                                     void* stream) {
+  if (!c || !dP || !dF || !d_thr_diag || !d_thr_nd || (mode != HFX_JK && mode != HFX_J_ONLY) ||
+      (flags & ~(HFX_BUILD_ZERO_F | HFX_BUILD_GRAPH)))
+    return HFX_E_ARG;
+  if (!c->have_schwarz) return HFX_E_STATE;
+  hipStream_t s = (hipStream_t)stream;
+  if (s && s != c->stream) {
+    hipEvent_t e;
+    HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPOK(hipEventRecord(e, s));
+    HIPOK(hipStreamWaitEvent(c->stream, e, 0));
+    hipEventDestroy(e);
+  }
+  GraphKey k;
+  k.P = dP;
+  k.F = dF;
+  k.thr_diag = d_thr_diag;
+  k.thr_nd = d_thr_nd;
+  k.split_mol = d_split;
+  k.split = d_split ? 0.0 : c->split;
+  k.mode = mode;
+  k.zero_f = (flags & HFX_BUILD_ZERO_F) ? 1 : 0;
+  const bool graph = (flags & HFX_BUILD_GRAPH) && !c->debug_blocks && !c->profile;
+  int r = graph ? graph_build(c, k) : build_full(c, k);
+  if (r) return r;
+  c->last_kind = 1;
+  if (s && s != c->stream) {
+    hipEvent_t e;
+    HIPOK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPOK(hipEventRecord(e, c->stream));
+    HIPOK(hipStreamWaitEvent(s, e, 0));
+    hipEventDestroy(e);
+  }
+  return HFX_OK;
+}
 
 extern "C" {
 
@@ -1244,7 +1555,7 @@ int hfx_get_stats(hfx_ctx* c, hfx_stats* out) {
   if (!c->last_kind) return HFX_OK;
   HIPOK(hipStreamSynchronize(c->stream));
   std::vector<unsigned long long> st(2 * HFX_NCLASSES * 3);
-  HIPOK(hipMemcpy(st.data(), c->d_stats, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  HIPOK(copy_sync(c->stream, st.data(), c->d_stats, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   for (int cl = 0; cl < HFX_NCLASSES; ++cl) {
     const ClassKernels& k = kClassKernels[cl];
     for (int p = 0; p < 2; ++p) {
@@ -1274,13 +1585,18 @@ int hfx_get_stats(hfx_ctx* c, hfx_stats* out) {
 }
 
 int hfx_get_schwarz(hfx_ctx* c, int t, double* schw, int* order, int* last) {
-  if (!c || t < 0 || t >= HFX_NPAIRTYPES) return HFX_E_ARG;
+  return hfx_batch_get_schwarz(c, 0, t, schw, order, last);
+}
+
+int hfx_batch_get_schwarz(hfx_ctx* c, int m, int t, double* schw, int* order, int* last) {
+  if (!c || t < 0 || t >= HFX_NPAIRTYPES || m < 0 || m >= c->nmol) return HFX_E_ARG;
   if (!c->have_schwarz) return HFX_E_STATE;
-  for (int k = 0; k < c->npair[t]; ++k) {
-    if (schw) schw[k] = c->schw[t][k];
-    if (order) order[k] = c->order[t][k] + 1;
+  const MolPart& mp = c->mols[m];
+  for (int k = 0; k < mp.npair[t]; ++k) {
+    if (schw) schw[k] = mp.schw[t][k];
+    if (order) order[k] = mp.order[t][k] + 1;
   }
-  if (last) *last = c->last[t];
+  if (last) *last = mp.last[t];
   return HFX_OK;
 }
 
@@ -1292,7 +1608,7 @@ long long hfx_debug_blocks(hfx_ctx* c, int cls, int prec, int* quartets, double*
     if (x.cls == cls) cp = &x;
   if (!cp) return 0;
   int n = 0;
-  if (hipMemcpy(&n, c->d_counts + prec * HFX_NCLASSES + cls, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+  if (copy_sync(c->stream, &n, c->d_counts + prec * HFX_NCLASSES + cls, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
     return HFX_E_HIP;
   if (n <= 0) return 0;
   const int2* list = nullptr;
@@ -1303,9 +1619,9 @@ long long hfx_debug_blocks(hfx_ctx* c, int cls, int prec, int* quartets, double*
     list = prec ? cp->list32 : cp->list64;
   }
   double* b = prec ? cp->blocks32 : cp->blocks64;
-  if (quartets && list && hipMemcpy(quartets, list, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost) != hipSuccess)
+  if (quartets && list && copy_sync(c->stream, quartets, list, (size_t)n * sizeof(int2), hipMemcpyDeviceToHost) != hipSuccess)
     return HFX_E_HIP;
-  if (blocks && b && hipMemcpy(blocks, b, (size_t)n * 625 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+  if (blocks && b && copy_sync(c->stream, blocks, b, (size_t)n * 625 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
     return HFX_E_HIP;
   return n;
 }

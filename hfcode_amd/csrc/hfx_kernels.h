@@ -7,31 +7,38 @@ namespace hfx {
 struct ScreenArgs {
   const int* bra;              // global pair index per bra list position
   const int* ket;
-  const int4* binfo;           // per list position: shell a, shell b, np, npall
+  const int4* binfo;           // per list position: shells a, b (numbered in the molecule), np, npall
   const int4* kinfo;
   const double* schw_bra;      // Schwarz value per list position
   const double* schw_ket;
   const PairRec* pairs;
-  const double* maxdm;         // nsh x nsh
+  const double* maxdm;         // per molecule an ns x ns block, md_stride doubles apart
+  long long md_stride;
+  const int* bmol;             // molecule per bra position (null: one molecule)
+  const int* kseg;             // first ket position of each molecule, nmol + 1 entries (null: [0, nket))
+  const double* thr_mol;       // threshold per molecule (null: thr)
+  const double* split_mol;     // FP32 split per molecule (null: split)
   int2* list64;
   int2* list32;
   int* count64;
   int* count32;
   unsigned long long* stats64; // [quartets, primitive quartets, bra primitives]
   unsigned long long* stats32;
-  const unsigned long long* pmax;  // max over maxdm, bits of a non-negative double
+  const unsigned long long* pmax;  // per molecule: max over its maxdm, bits of a non-negative double
   unsigned long long* masks;   // survivor bits: [bra][chunk of HFX_WAVE kets][list 64, list 32]
   double thr;
   double split;                // > 0: bound < split -> FP32 list
-  int nbra, nket, same, jonly, nsh, all_prims;
+  int nbra, nket, same, jonly, nsh, all_prims;   // nsh: row length of a molecule's maxdm block
   int nchunk;                  // chunks per bra row of masks
-  int ket_sorted;              // schw_ket is non-increasing (the early stop applies)
+  int ket_sorted;              // schw_ket is non-increasing within each molecule (the early stop applies)
 };
 
 struct DiagScreenArgs {
   const PairRec* pairs;
   const double* schw;          // Schwarz value per pair of the type (list order)
-  const double* maxdm;
+  const double* maxdm;         // per molecule an nsh x nsh block, md_stride doubles apart
+  long long md_stride;
+  const double* thr_mol;       // threshold per molecule (null: thr)
   int2* list;
   int* count;
   unsigned long long* stats;
@@ -61,6 +68,7 @@ struct DigestArgs {
   int tb_offset;               // byte offset in LDS of the per-wave staging of X_cd (after the rows)
   int dims[4];                 // components of the oriented shells A, B, C, D
   int swap_bra, swap_ket;      // oriented A is the pair's second shell (bra / ket)
+  const double* thr_mol;       // block test threshold per molecule (device), or null: thr
 };
 
 // One-electron kernels (generated, gen/onee.hip): one thread per work item
@@ -93,11 +101,11 @@ extern const OneeKernel kOneeV[3][3];
 __global__ void k_fill_f64(double* x, size_t n, double v);
 __global__ void k_fill_i32(int* x, int n, int v);
 __global__ void k_fill_u64(unsigned long long* x, int n);
-__global__ void k_maxdm(const double* P, int nb, const ShellRec* sh, int nsh, double* maxdm,
+__global__ void k_maxdm(const double* P, int nb, const ShellRec* sh, const int2* molsh, int nsp, double* maxdm,
                         unsigned long long* pmax);
 __global__ void k_screen(const ScreenArgs s);
 __global__ void k_screen_diag(const DiagScreenArgs s);
-__global__ void k_add_sym(const double* X, double* F, int nb);
+__global__ void k_add_sym(const double* X, double* F, int nb, int nmat);
 __global__ void k_to_f32(const double* x, float* y, size_t n);
 // X += the pair buffer, block by block (every global pair once)
 __global__ void k_pairbuf_to_x(const double* pairbuf, const PairRec* pairs, const ShellRec* shells, int npairs,

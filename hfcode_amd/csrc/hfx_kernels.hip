@@ -17,26 +17,35 @@ __global__ void k_fill_u64(unsigned long long* x, int n) {
 }
 
 // maxdm(a,b) = max |P| over the AO block of shell pair (a,b), stored as a
-// symmetric nsh x nsh matrix.  The block is the one evaluate_maxdm reads
-// (evaluate_maxdm.f90:1-101): rows = AOs of the first shell of the pair in
-// list order (s before p before d before L), columns = AOs of the second.
-// For a (d, L) pair the reference reads rows ao_d..ao_d+3 and columns
-// ao_L..ao_L+4 (evaluate_maxdm.f90:82-90): 4 of the 5 d functions and the AO
-// after the L shell.  That block is reproduced so that screening decisions
-// match the reference exactly; a column past nb is clamped.
+// symmetric nsp x nsp block per molecule (blockIdx.y = molecule; molsh[m] =
+// its first shell and shell count; P block of molecule m at m * nb * nb).
+// The block is the one evaluate_maxdm reads (evaluate_maxdm.f90:1-101):
+// rows = AOs of the first shell of the pair in list order (s before p before
+// d before L), columns = AOs of the second.  For a (d, L) pair the reference
+// reads rows ao_d..ao_d+3 and columns ao_L..ao_L+4 (evaluate_maxdm.f90:82-90):
+// 4 of the 5 d functions and the AO after the L shell.  That block is
+// reproduced so that screening decisions match the reference exactly; a
+// column past nb is clamped.
 //
-// pmax receives max over all entries (the bits of a non-negative double,
-// whose integer order is the numeric order); it bounds every screening
-// factor, which lets the screening stop early along a Schwarz-sorted ket list.
-__global__ void k_maxdm(const double* __restrict__ P, int nb, const ShellRec* __restrict__ sh, int nsh,
-                        double* __restrict__ maxdm, unsigned long long* __restrict__ pmax) {
+// pmax[m] receives the maximum over molecule m's entries (the bits of a
+// non-negative double, whose integer order is the numeric order); it bounds
+// every screening factor, which lets the screening stop early along a
+// Schwarz-sorted ket list.
+__global__ void k_maxdm(const double* __restrict__ P, int nb, const ShellRec* __restrict__ sh,
+                        const int2* __restrict__ molsh, int nsp, double* __restrict__ maxdm,
+                        unsigned long long* __restrict__ pmax) {
+  const int mol = blockIdx.y;
+  const int2 ms = molsh[mol];
+  const int nsh = ms.y;
+  const double* Pm = P + (size_t)mol * nb * nb;
+  double* md = maxdm + (size_t)mol * nsp * nsp;
   const long long n = (long long)nsh * nsh;
   double tmax = 0.0;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
        idx += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(idx / nsh), j = (int)(idx - (long long)i * nsh);
     if (j > i) continue;
-    ShellRec s1 = sh[i], s2 = sh[j];
+    ShellRec s1 = sh[ms.x + i], s2 = sh[ms.x + j];
     // list orientation: canonical letter order s < p < d < L
     auto rank = [](int t) { return t == 1 ? 0 : t == 2 ? 1 : t == 3 ? 2 : 3; };
     if (rank(s2.type) < rank(s1.type)) { ShellRec t = s1; s1 = s2; s2 = t; }
@@ -48,15 +57,15 @@ __global__ void k_maxdm(const double* __restrict__ P, int nb, const ShellRec* __
       for (int c = 0; c < nc; ++c) {
         int ac = s2.ao + c;
         if (ac >= nb) ac = nb - 1;
-        m = fmax(m, fabs(P[(size_t)ar * nb + ac]));
+        m = fmax(m, fabs(Pm[(size_t)ar * nb + ac]));
       }
     }
-    maxdm[(size_t)i * nsh + j] = m;
-    maxdm[(size_t)j * nsh + i] = m;
+    md[(size_t)i * nsp + j] = m;
+    md[(size_t)j * nsp + i] = m;
     tmax = fmax(tmax, m);
   }
   for (int o = HFX_WAVE / 2; o > 0; o >>= 1) tmax = fmax(tmax, __shfl_xor(tmax, o));
-  if (__lane_id() == 0) atomicMax(pmax, (unsigned long long)__double_as_longlong(tmax));
+  if (__lane_id() == 0) atomicMax(pmax + mol, (unsigned long long)__double_as_longlong(tmax));
 }
 
 // Wave-aggregated append of (bra, ket) to a list: one atomic per wave.
@@ -112,19 +121,25 @@ __device__ inline unsigned long long wave_sum(unsigned long long a) {
 __global__ void k_screen(const ScreenArgs s) {
   const int lane = __lane_id();
   const int nw = gridDim.x * (blockDim.x / HFX_WAVE);
-  const double pm = __longlong_as_double((long long)*s.pmax);
   const int ns = s.nsh;
-  const double* md = s.maxdm;
   for (int i = blockIdx.x * (blockDim.x / HFX_WAVE) + threadIdx.x / HFX_WAVE; i < s.nbra; i += nw) {
-    const int jend = s.same ? min(i, s.nket) : s.nket;
+    // the bra's molecule: its kets, maxdm block, density maximum and threshold
+    const int mol = s.bmol ? s.bmol[i] : 0;
+    const int kb = s.kseg ? s.kseg[mol] : 0;
+    const int ke = s.kseg ? s.kseg[mol + 1] : s.nket;
+    const double* md = s.maxdm + mol * s.md_stride;
+    const double pm = __longlong_as_double((long long)s.pmax[mol]);
+    const double thr = s.thr_mol ? s.thr_mol[mol] : s.thr;
+    const double split = s.split_mol ? s.split_mol[mol] : s.split;
+    const int jend = s.same ? min(i, ke) : ke;
     const double sb = s.schw_bra[i];
     // first ket position whose bound cannot reach thr
-    int lo = 0, hi = jend;
+    int lo = kb, hi = jend;
     if (s.ket_sorted) {
       while (hi - lo > HFX_WAVE) {
         const int step = (hi - lo + HFX_WAVE - 1) / HFX_WAVE;
         const int p = lo + lane * step;
-        const bool keep = p < hi && !(sb * s.schw_ket[p] * pm < s.thr);
+        const bool keep = p < hi && !(sb * s.schw_ket[p] * pm < thr);
         const int c = __popcll(__ballot(keep));          // the kept probes are a prefix
         const int pc = lo + c * step;                    // first failing probe, if any
         const int nlo = c == 0 ? lo : lo + (c - 1) * step + 1;
@@ -132,7 +147,7 @@ __global__ void k_screen(const ScreenArgs s) {
         lo = nlo;
       }
       const int p = lo + lane;
-      const bool keep = p < hi && !(sb * s.schw_ket[p] * pm < s.thr);
+      const bool keep = p < hi && !(sb * s.schw_ket[p] * pm < thr);
       hi = lo + __popcll(__ballot(keep));
     }
     const int jcut = hi;
@@ -143,7 +158,7 @@ __global__ void k_screen(const ScreenArgs s) {
     unsigned long long pq64 = 0, bp64 = 0, pq32 = 0, bp32 = 0;
     unsigned long long klo = 0, khi = 0;   // set of survivors' primitive counts + 1 (<= 101)
     #pragma unroll 4
-    for (int j0 = 0; j0 < jcut; j0 += HFX_WAVE) {
+    for (int j0 = kb; j0 < jcut; j0 += HFX_WAVE) {
       const int j = j0 + lane;
       bool k64 = false, k32 = false;
       if (j < jcut) {
@@ -155,8 +170,8 @@ __global__ void k_screen(const ScreenArgs s) {
           mv = fmax(mv, 0.25 * ex);
         }
         const double bound = sb * s.schw_ket[j] * mv;
-        if (!(bound < s.thr)) {
-          if (s.split > 0.0 && bound < s.split) k32 = true; else k64 = true;
+        if (!(bound < thr)) {
+          if (split > 0.0 && bound < split) k32 = true; else k64 = true;
           const unsigned long long nb_ = s.all_prims ? b.w : b.z;
           const unsigned long long nk_ = s.all_prims ? k.w : k.z;
           if (k64) { pq64 += nb_ * nk_; bp64 += nb_; } else { pq32 += nb_ * nk_; bp32 += nb_; }
@@ -166,8 +181,8 @@ __global__ void k_screen(const ScreenArgs s) {
       }
       const unsigned long long m64 = __ballot(k64), m32 = __ballot(k32);
       if (lane == 0) {
-        mrow[(j0 / HFX_WAVE) * 2] = m64;
-        mrow[(j0 / HFX_WAVE) * 2 + 1] = m32;
+        mrow[((j0 - kb) / HFX_WAVE) * 2] = m64;
+        mrow[((j0 - kb) / HFX_WAVE) * 2 + 1] = m32;
       }
       n64 += __popcll(m64);
       n32 += __popcll(m32);
@@ -205,9 +220,9 @@ __global__ void k_screen(const ScreenArgs s) {
       const int key = khi ? 127 - __clzll((long long)khi) : 63 - __clzll((long long)klo);
       if (key >= 64) khi &= ~(1ull << (key - 64)); else klo &= ~(1ull << key);
       #pragma unroll 4
-      for (int j0 = 0; j0 < jcut; j0 += HFX_WAVE) {
-        const unsigned long long m64 = mrow[(j0 / HFX_WAVE) * 2];
-        const unsigned long long m32 = mrow[(j0 / HFX_WAVE) * 2 + 1];
+      for (int j0 = kb; j0 < jcut; j0 += HFX_WAVE) {
+        const unsigned long long m64 = mrow[((j0 - kb) / HFX_WAVE) * 2];
+        const unsigned long long m32 = mrow[((j0 - kb) / HFX_WAVE) * 2 + 1];
         bool sel = false;
         if ((m64 | m32) >> lane & 1ull) {
           const int4 k = s.kinfo[j0 + lane];
@@ -241,11 +256,11 @@ __global__ void k_screen_diag(const DiagScreenArgs s) {
       const PairRec b = s.pairs[g];
       if (s.check) {
         const int ns = s.nsh;
-        const double mv = fmax(s.maxdm[(size_t)b.sha * ns + b.shb],
-                               fmax(0.25 * s.maxdm[(size_t)b.shb * ns + b.shb],
-                                    0.25 * s.maxdm[(size_t)b.sha * ns + b.sha]));
+        const double* md = s.maxdm + b.mol * s.md_stride;
+        const double mv = fmax(md[(size_t)b.lsa * ns + b.lsb],
+                               fmax(0.25 * md[(size_t)b.lsb * ns + b.lsb], 0.25 * md[(size_t)b.lsa * ns + b.lsa]));
         const double sc = s.schw[i];
-        if (sc * sc * mv < s.thr) ok = false;
+        if (sc * sc * mv < (s.thr_mol ? s.thr_mol[b.mol] : s.thr)) ok = false;
       }
       if (ok) {
         const int np_ = s.all_prims ? b.npall : b.np;
@@ -257,7 +272,6 @@ __global__ void k_screen_diag(const DiagScreenArgs s) {
   }
 }
 
-// F += X + X^T (the accumulator holds each symmetric contribution once).
 // X += pair buffer: one thread per (pair, element); the pairs are distinct
 // shell pairs, so every element of X is written by at most one thread.
 __global__ void k_pairbuf_to_x(const double* __restrict__ pairbuf, const PairRec* __restrict__ pairs,
@@ -270,7 +284,7 @@ __global__ void k_pairbuf_to_x(const double* __restrict__ pairbuf, const PairRec
     const PairRec& pr = pairs[p];
     const int nbc = shells[pr.shb].ncomp;
     const int ia = e / nbc, ib = e - ia * nbc;
-    if (ia < shells[pr.sha].ncomp) X[(size_t)(pr.aoa + ia) * nb + pr.aob + ib] += v;
+    if (ia < shells[pr.sha].ncomp) X[pr.mbase + (size_t)(pr.aoa + ia) * nb + pr.aob + ib] += v;
   }
 }
 
@@ -280,12 +294,15 @@ __global__ void k_to_f32(const double* __restrict__ x, float* __restrict__ y, si
     y[i] = (float)x[i];
 }
 
-__global__ void k_add_sym(const double* __restrict__ X, double* __restrict__ F, int nb) {
-  const long long n = (long long)nb * nb;
+// F += X + X^T per nb x nb matrix (the accumulator holds each symmetric
+// contribution once); nmat stacked matrices.
+__global__ void k_add_sym(const double* __restrict__ X, double* __restrict__ F, int nb, int nmat) {
+  const long long n2 = (long long)nb * nb, n = n2 * nmat;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
        idx += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(idx / nb), j = (int)(idx - (long long)i * nb);
-    F[idx] += X[idx] + X[(size_t)j * nb + i];
+    const long long m0 = idx / n2 * n2, r = idx - m0;   // matrix, element within it
+    const int i = (int)(r / nb), j = (int)(r - (long long)i * nb);
+    F[idx] += X[idx] + X[m0 + (size_t)j * nb + i];
   }
 }
 

@@ -138,7 +138,7 @@ __global__ void k_scale_cols(const double* V, const double* d, double* W, int nb
 std::mutex g_handles_mu;
 std::vector<rocblas_handle>* g_handles = new std::vector<rocblas_handle>;   // lives until the process ends
 
-rocblas_handle take_handle() {
+rocblas_handle take_handle_impl() {
   {
     std::lock_guard<std::mutex> lk(g_handles_mu);
     if (!g_handles->empty()) {
@@ -148,13 +148,24 @@ rocblas_handle take_handle() {
     }
   }
   rocblas_handle h = nullptr;
+  hfx::DeviceSection ds;   // allocates the handle's workspace
   return rocblas_create_handle(&h) == rocblas_status_success ? h : nullptr;
 }
 
-void give_handle(rocblas_handle h) {
+void give_handle_impl(rocblas_handle h) {
   std::lock_guard<std::mutex> lk(g_handles_mu);
   g_handles->push_back(h);
 }
+
+}  // namespace
+
+void* hfx::take_blas_handle() { return take_handle_impl(); }
+void hfx::give_blas_handle(void* h) { give_handle_impl((rocblas_handle)h); }
+
+namespace {
+
+rocblas_handle take_handle() { return take_handle_impl(); }
+void give_handle(rocblas_handle h) { give_handle_impl(h); }
 
 struct Solver {
   hfx_ctx* ctx = nullptr;
@@ -190,11 +201,13 @@ struct Solver {
   bool gemm(const double* a, bool ta, const double* b, bool tb, double* c, double alpha = 1.0, double beta = 0.0,
             int k = -1) {
     const int kk = k < 0 ? nb : k;
+    hfx::DeviceSection ds;   // a call may resize the handle's workspace
     return rocblas_dgemm(hb, ta ? rocblas_operation_transpose : rocblas_operation_none,
                          tb ? rocblas_operation_transpose : rocblas_operation_none, nb, nb, kk, &alpha, a, nb, b,
                          nb, &beta, c, nb) == rocblas_status_success;
   }
   bool eigh(double* a_inout, double* w) {
+    hfx::DeviceSection ds;
     return rocsolver_dsyevd(hb, rocblas_evect_original, rocblas_fill_upper, nb, a_inout, nb, w, E, info) ==
            rocblas_status_success;
   }

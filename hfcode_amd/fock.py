@@ -133,6 +133,17 @@ def configure(L):
     L.hfx_ctx_stream.argtypes = [vp]
     L.hfx_ctx_stream.restype = vp
     L.hfx_ctx_nb.argtypes = [vp]
+    L.hfx_ctx_nmol.argtypes = [vp]
+    L.hfx_batch_create.argtypes = [ctypes.POINTER(MolDesc), ctypes.c_int, ctypes.POINTER(vp)]
+    L.hfx_batch_get_schwarz.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ip, ip]
+    L.hfx_batch_fock_device.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
+    L.hfx_batch_onee.argtypes = [vp, ip, dp, dp, dp, dp, dp]
+    L.hfx_batch_scf_create.argtypes = [vp, dp, dp, ip, dp, dp, ctypes.POINTER(ScfOptions), ctypes.c_int,
+                                       ctypes.POINTER(vp)]
+    L.hfx_batch_scf_step.argtypes = [vp, ip, dp, ip]
+    L.hfx_batch_scf_result.argtypes = [vp, ctypes.c_int, dp, dp, dp]
+    L.hfx_batch_scf_destroy.argtypes = [vp]
+    L.hfx_batch_scf_destroy.restype = None
     L.hfx_scf_defaults.argtypes = [ctypes.POINTER(ScfOptions)]
     L.hfx_scf_defaults.restype = None
     L.hfx_scf_run.argtypes = [vp, dp, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.POINTER(ScfOptions),
@@ -457,3 +468,164 @@ class FockBuilder:
         t, i = ctypes.c_int(), ctypes.c_int()
         _check(lib().hfx_pair_info(self._h, int(g), ctypes.byref(t), ctypes.byref(i)))
         return t.value, i.value
+
+
+BUILD_ZERO_F, BUILD_GRAPH = 1, 2
+
+
+class BatchBuilder:
+    """Several molecules in one context (``hfx_batch_create``): each quartet
+    class is screened and evaluated for all of them in one launch of its
+    kernels.  Matrices are stacked ``(nmol, nb, nb)`` arrays with ``nb`` the
+    largest molecule's basis size; molecule m uses ``[m, :nb_m, :nb_m]``
+    (:meth:`stack` / :meth:`unstack`).  The Fock-build methods take and
+    return such stacks, with the meaning of :class:`FockBuilder`'s."""
+
+    def __init__(self, shell_tables):
+        L = lib()
+        n = len(shell_tables)
+        if n < 1:
+            raise ValueError("a batch needs at least one molecule")
+        self._descs = (MolDesc * n)()
+        self._keep = []
+        self.nbs, self.pair_lists = [], []
+        for i, st in enumerate(shell_tables):
+            d, keep, nb, lists = build_desc(st)
+            self._descs[i] = d
+            self._keep.append(keep)
+            self.nbs.append(nb)
+            self.pair_lists.append(lists)
+        h = ctypes.c_void_p()
+        _check(L.hfx_batch_create(self._descs, n, ctypes.byref(h)))
+        self._h = h
+        self.nmol = n
+        self.nb = L.hfx_ctx_nb(h)
+        self.split = 0.0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hfx_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stack(self, mats):
+        """(nmol, nb, nb) array from per-molecule matrices (zero padding)."""
+        out = np.zeros((self.nmol, self.nb, self.nb))
+        for m, a in enumerate(mats):
+            a = np.asarray(a, dtype=np.float64)
+            out[m, :a.shape[0], :a.shape[1]] = a
+        return out
+
+    def unstack(self, arr, m):
+        n = self.nbs[m]
+        return np.array(arr[m, :n, :n])
+
+    def set_mixed(self, split):
+        _check(lib().hfx_set_option(self._h, OPT_MIXED_SPLIT, float(split)))
+        self.split = float(split)
+
+    def onee(self, mols):
+        """Stacked (S, T, V) of the molecules (objects with znuc, coords_bohr)."""
+        nat = np.array([len(m.znuc) for m in mols], dtype=np.int32)
+        ch = np.ascontiguousarray(np.concatenate([np.asarray(m.znuc, dtype=np.float64) for m in mols]))
+        xy = np.ascontiguousarray(np.concatenate([np.asarray(m.coords_bohr, dtype=np.float64).reshape(-1, 3)
+                                                  for m in mols]))
+        S, T, V = (np.zeros((self.nmol, self.nb, self.nb)) for _ in range(3))
+        _check(lib().hfx_batch_onee(self._h, _ip(nat), _dp(ch), _dp(xy), _dp(S), _dp(T), _dp(V)))
+        return S, T, V
+
+    def _prep(self, f, p):
+        shape = (self.nmol, self.nb, self.nb)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        f = np.array(f, dtype=np.float64, copy=True, order="C")
+        if p.shape != shape or f.shape != shape:
+            raise ValueError("P and F must be %r" % (shape,))
+        return f, p
+
+    def non_diag(self, f, p, thr, mode="jk"):
+        f, p = self._prep(f, p)
+        _check(lib().hfx_fock_nd(self._h, _dp(p), _dp(f), self.nb, float(thr), HFX_J_ONLY if mode == "j" else HFX_JK))
+        return f
+
+    def diag(self, f, p, thr):
+        f, p = self._prep(f, p)
+        _check(lib().hfx_fock_diag(self._h, _dp(p), _dp(f), self.nb, float(thr)))
+        return f
+
+    def initial_diag(self, f, p, thr):
+        f, p = self._prep(f, p)
+        _check(lib().hfx_fock_initial_diag(self._h, _dp(p), _dp(f), self.nb, float(thr)))
+        return f
+
+    def build_device(self, p_ptr, f_ptr, thr_diag_ptr, thr_nd_ptr, split_ptr=None, mode="jk", zero_f=True,
+                     graph=True, stream=None):
+        """hfx_batch_fock_device on device arrays (raw pointers): F (+)= the
+        diag + non-diag build of every molecule at its own thresholds."""
+        flags = (BUILD_ZERO_F if zero_f else 0) | (BUILD_GRAPH if graph else 0)
+        _check(lib().hfx_batch_fock_device(self._h, ctypes.c_void_p(p_ptr), ctypes.c_void_p(f_ptr),
+                                           ctypes.c_void_p(thr_diag_ptr), ctypes.c_void_p(thr_nd_ptr),
+                                           ctypes.c_void_p(split_ptr or 0), HFX_J_ONLY if mode == "j" else HFX_JK,
+                                           flags, ctypes.c_void_p(stream or 0)))
+
+    def schwarz(self, m):
+        out = []
+        for t, prs in enumerate(self.pair_lists[m]):
+            n = len(prs)
+            sc = np.zeros(max(n, 1)); o = np.zeros(max(n, 1), np.int32); last = ctypes.c_int(0)
+            _check(lib().hfx_batch_get_schwarz(self._h, m, t, _dp(sc), _ip(o), ctypes.byref(last)))
+            out.append((prs, sc[:n], o[:n], last.value))
+        return out
+
+    def scf(self, S, H, nocc, unuc, P0=None, maxiter=25, ndiis=8, conv_e=5e-8, conv_p=1e-5, lowthresh=5e-7,
+            intthresh=1e-10, fdifffac=0.1, diis_start=0.3, mode="jk", use_graphs=True, on_done=None):
+        """Direct RHF of every molecule at once on the GPU (hfx_batch_scf_*).
+        S, H, P0: stacked arrays; nocc, unuc: one per molecule.  `on_done(m,
+        result)` is called as each molecule finishes.  Returns a list of
+        SimpleNamespace(energy, P, F, eigs, iterations, converged, status)."""
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        H = np.ascontiguousarray(H, dtype=np.float64)
+        p0 = None if P0 is None else np.ascontiguousarray(P0, dtype=np.float64)
+        occ = np.ascontiguousarray(nocc, dtype=np.int32)
+        un = np.ascontiguousarray(unuc, dtype=np.float64)
+        o = ScfOptions()
+        L = lib()
+        L.hfx_scf_defaults(ctypes.byref(o))
+        o.maxiter, o.ndiis, o.conv_e, o.conv_p = int(maxiter), int(ndiis), float(conv_e), float(conv_p)
+        o.lowthresh, o.intthresh, o.fdifffac = float(lowthresh), float(intthresh), float(fdifffac)
+        o.diis_start = min(float(diis_start), 1e300)
+        o.levelshift = 0.0
+        o.mode = HFX_J_ONLY if mode == "j" else HFX_JK
+        h = ctypes.c_void_p()
+        _check(L.hfx_batch_scf_create(self._h, _dp(S), _dp(H), _ip(occ), _dp(un),
+                                      _dp(p0) if p0 is not None else None, ctypes.byref(o), int(bool(use_graphs)),
+                                      ctypes.byref(h)))
+        n = self.nmol
+        status = np.zeros(n, np.int32)
+        energy = np.zeros(n)
+        iters = np.zeros(n, np.int32)
+        results = [None] * n
+        try:
+            while True:
+                live = L.hfx_batch_scf_step(h, _ip(status), _dp(energy), _ip(iters))
+                if live < 0:
+                    _check(live)
+                for m in range(n):
+                    if results[m] is None and status[m] != 0:
+                        nbm = self.nbs[m]
+                        P, F, e = np.zeros((nbm, nbm)), np.zeros((nbm, nbm)), np.zeros(nbm)
+                        _check(L.hfx_batch_scf_result(h, m, _dp(P), _dp(F), _dp(e)))
+                        results[m] = SimpleNamespace(energy=float(energy[m]), P=P, F=F, eigs=e,
+                                                     iterations=int(iters[m]), converged=bool(status[m] == 1),
+                                                     status=int(status[m]))
+                        if on_done is not None:
+                            on_done(m, results[m])
+                if live == 0:
+                    break
+        finally:
+            L.hfx_batch_scf_destroy(h)
+        return results

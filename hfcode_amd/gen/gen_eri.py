@@ -216,6 +216,9 @@ class Gen:
         W("    const int shA = pb.sh%s, shB = pb.sh%s, shC = pk.sh%s, shD = pk.sh%s;" % (oa, ob, oc, od))
         W("    const int aoA = pb.ao%s, aoB = pb.ao%s, aoC = pk.ao%s, aoD = pk.ao%s;" % (oa, ob, oc, od))
         W("    (void)shA; (void)shB; (void)shC; (void)shD; (void)aoA; (void)aoB; (void)aoC; (void)aoD;")
+        # the quartet's molecule: its P / X block starts at mbase (stacked nb x nb matrices
+        # of a batch context; 0 with one molecule) and its threshold may be its own
+        W("    const long long mq = pb.mbase; (void)mq;")
 
     def emit_group(self, W, combos, ops_prim, ops_bra, ops_ctr, gid, sink=None):
         """Primitive loops + contraction + HRR + spherical transform for the
@@ -622,6 +625,7 @@ class Gen:
         W("  __syncthreads();   // rows zeroed before any wave adds to them")
         W("  const int sstride = rows ? 1 : gridDim.x;")
         W("  int curb = -1, rA = 0, rB = 0;   // resident bra pair and its rows")
+        W("  double* rX = X;                  // its molecule's block of X")
         W("  auto flush = [&]() {")
         W("    __syncthreads();")
         W("    for (int e = threadIdx.x; e < %d * nbf; e += blockDim.x) {" % (nA + nB))
@@ -629,7 +633,7 @@ class Gen:
         W("      if (v != 0.0) {")
         W("        const int r = e / nbf, col = e - r * nbf;")
         W("        const int row = r < %d ? rA + r : rB + (r - %d);" % (nA, nA))
-        W("        atomicAdd(&X[(size_t)row * nbf + col], v);")
+        W("        atomicAdd(&rX[(size_t)row * nbf + col], v);")
         W("        srow[e] = 0.0;")
         W("      }")
         W("    }")
@@ -642,6 +646,7 @@ class Gen:
         W("        if (curb >= 0) flush();")
         W("        curb = tb;")
         W("        rA = a.pairs[tb].ao%s; rB = a.pairs[tb].ao%s;" % (oa, ob))
+        W("        rX = X + a.pairs[tb].mbase;")
         W("      }")
         W("    }")
         W("    const int it = st * blockDim.x + (threadIdx.x - lane);   // first quartet of the wave")
@@ -658,7 +663,7 @@ class Gen:
         def pblk(name, ao1, n1, ao2, n2, konly):
             for i in range(n1):
                 for j in range(n2):
-                    ld = "a.P[(size_t)(%s + %d) * nbf + %s + %d]" % (ao1, i, ao2, j)
+                    ld = "a.P[mq + (size_t)(%s + %d) * nbf + %s + %d]" % (ao1, i, ao2, j)
                     if konly:
                         ld = "JONLY ? 0.0 : " + ld
                     W("    const double %s_%d_%d = %s;" % (name, i, j, ld))
@@ -698,7 +703,8 @@ class Gen:
             W("    }")
             ops_ctr.n += 12 + 1
         W("    if (a.schw_out && valid) a.schw_out[bk.x] = sqrt((double)vmax);")
-        W("    const bool act = valid && (a.diag || (double)vmax >= a.thr);")
+        W("    const double thq = a.thr_mol ? a.thr_mol[pb.mol] : a.thr;")
+        W("    const bool act = valid && (a.diag || (double)vmax >= thq);")
         W("    if (__ballot(act) == 0) continue;")
         W("    const bool inl = rows && act && bk.x == curb;   // rows A, B resident in LDS")
         W("    const double w = act ? ((shA == shB) ? 0.5 : 1.0) * ((shC == shD) ? 0.5 : 1.0) * (a.diag ? 0.5 : 1.0) : 0.0;")
@@ -712,7 +718,7 @@ class Gen:
                 W("        for (int o = HFX_WAVE / 2; o > 0; o >>= 1) r += __shfl_xor(r, o);")
                 W("        if (lane == 0 && r != 0.0) atomicAdd(&srow[%d * nbf + rB + %d], r);" % (i, j))
                 W("      }")
-                W("      if (act && !inl) atomicAdd(&X[(size_t)(aoA + %d) * nbf + aoB + %d], jab);" % (i, j))
+                W("      if (act && !inl) atomicAdd(&X[mq + (size_t)(aoA + %d) * nbf + aoB + %d], jab);" % (i, j))
                 W("    }")
         # J_cd into the ket pair's contiguous pair-buffer slot (the pair's own shell order)
         W("    if (act) {")
@@ -731,7 +737,7 @@ class Gen:
             W("      } else if (act) {")
             for i in range(n1):
                 for j in range(n2):
-                    W("        atomicAdd(&X[(size_t)(%s + %d) * nbf + %s + %d], wk * %s_%d_%d);" % (ao1, i, ao2, j, name, i, j))
+                    W("        atomicAdd(&X[mq + (size_t)(%s + %d) * nbf + %s + %d], wk * %s_%d_%d);" % (ao1, i, ao2, j, name, i, j))
             W("      }")
 
         W("    if constexpr (!JONLY) {")
@@ -763,6 +769,7 @@ class Gen:
             W("template <typename R>")
             W("__global__ void __launch_bounds__(HFX_NT) eri_%s_%s%d(const ClassArgs a) {" % (cls, kname, gi))
             W("  extern __shared__ char hfx_smem[];")
+            W("  if (a.q0 >= *a.count) return;   // a chunk past the list's end (chunks sized by capacity)")
             W("  R* btab = reinterpret_cast<R*>(hfx_smem);")
             W("  constexpr int NB_COL = %d + BoysTerms<R>::K;" % M)
             W("  stage_boys<R>(btab, a, NB_COL);")
@@ -826,7 +833,7 @@ class Gen:
         def pload(name, ao1, n1, ao2, n2):
             for i in range(n1):
                 for j in range(n2):
-                    W("      const T %s_%d_%d = PA[(size_t)(%s + %d) * nbf + %s + %d];" % (name, i, j, ao1, i, ao2, j))
+                    W("      const T %s_%d_%d = PA[mq + (size_t)(%s + %d) * nbf + %s + %d];" % (name, i, j, ao1, i, ao2, j))
 
         def zero(name, n):
             W("      " + " ".join("T %s_%d = 0;" % (name, i) for i in range(n)))
@@ -842,7 +849,7 @@ class Gen:
                 W("        atomicAdd(&srow[%d * nbf + %s + %d], %s * (double)%s_%d);" % (row, ao2, j, wname, name, j))
             W("      } else if (act) {")
             for j in range(n):
-                W("        atomicAdd(&X[(size_t)(%s + %d) * nbf + %s + %d], %s * (double)%s_%d);" % (ao1, i, ao2, j, wname, name, j))
+                W("        atomicAdd(&X[mq + (size_t)(%s + %d) * nbf + %s + %d], %s * (double)%s_%d);" % (ao1, i, ao2, j, wname, name, j))
             W("      }")
 
         W("template <bool JONLY, typename T>")
@@ -872,6 +879,7 @@ class Gen:
         W("  __syncthreads();   // rows zeroed before any wave adds to them")
         W("  const int sstride = rows ? 1 : gridDim.x;")
         W("  int curb = -1, rA = 0, rB = 0;   // resident bra pair and its rows")
+        W("  double* rX = X;                  // its molecule's block of X")
         W("  auto flush = [&]() {")
         W("    __syncthreads();")
         W("    for (int e = threadIdx.x; e < %d * nbf; e += blockDim.x) {" % (nA + nB))
@@ -879,7 +887,7 @@ class Gen:
         W("      if (v != 0.0) {")
         W("        const int r = e / nbf, col = e - r * nbf;")
         W("        const int row = r < %d ? rA + r : rB + (r - %d);" % (nA, nA))
-        W("        atomicAdd(&X[(size_t)row * nbf + col], v);")
+        W("        atomicAdd(&rX[(size_t)row * nbf + col], v);")
         W("        srow[e] = 0.0;")
         W("      }")
         W("    }")
@@ -894,6 +902,7 @@ class Gen:
         W("        if (curb >= 0) flush();")
         W("        curb = tb;")
         W("        rA = d.pairs[tb].ao%s; rB = d.pairs[tb].ao%s;" % (oa, ob))
+        W("        rX = X + d.pairs[tb].mbase;")
         W("      }")
         W("    }")
         W("    if (it >= qend) continue;")
@@ -904,7 +913,8 @@ class Gen:
         W("    double vmax = d.vmax[slot];")
         W("    for (int c = 1; c < d.ncombo; ++c) vmax = fmax(vmax, d.vmax[(size_t)c * d.vstride + slot]);")
         W("    if (d.schw_out != nullptr && valid) d.schw_out[bk.x] = sqrt(vmax);")
-        W("    const bool act = valid && (d.diag || vmax >= d.thr);")
+        W("    const double thq = d.thr_mol ? d.thr_mol[d.pairs[bk.x].mol] : d.thr;")
+        W("    const bool act = valid && (d.diag || vmax >= thq);")
         W("    const T* blk = reinterpret_cast<const T*>(d.eblk) + (size_t)(slot >> 6) * %d + (slot & 63);" % (ne * 64))
         W("    if (d.dump != nullptr && valid) {")
         W("      #pragma unroll 1")
@@ -925,6 +935,7 @@ class Gen:
         W("    const PairRec& pk = d.pairs[bk.y];")
         W("    const int shA = pb.sh%s, shB = pb.sh%s, shC = pk.sh%s, shD = pk.sh%s;" % (oa, ob, oc, od))
         W("    const int aoA = pb.ao%s, aoB = pb.ao%s, aoC = pk.ao%s, aoD = pk.ao%s;" % (oa, ob, oc, od))
+        W("    const long long mq = pb.mbase;")
         W("    const double w = act ? ((shA == shB) ? 0.5 : 1.0) * ((shC == shD) ? 0.5 : 1.0) * (d.diag ? 0.5 : 1.0) : 0.0;")
         W("    const double wj = 2.0 * w, wk = -0.5 * w;")
         ncd = nC * nD
@@ -937,7 +948,7 @@ class Gen:
             W("        for (int o = HFX_WAVE / 2; o > 0; o >>= 1) r += __shfl_xor(r, o);")
             W("        if (lane == 0 && r != 0.0) atomicAdd(&srow[%d * nbf + rB + %d], r);" % (ia, ib))
             W("      }")
-            W("      if (act && !inl) atomicAdd(&X[(size_t)(aoA + %d) * nbf + aoB + %d], jd);" % (ia, ib))
+            W("      if (act && !inl) atomicAdd(&X[mq + (size_t)(aoA + %d) * nbf + aoB + %d], jd);" % (ia, ib))
 
         # Pass J, then K below; each pass re-derives the block pointer
         # so that element addresses are not shared, and kept live, across
@@ -953,7 +964,7 @@ class Gen:
             W("      {")
             for ib in range(nB):
                 W("      {")
-                W("      const T pab = PA[(size_t)(aoA + %d) * nbf + aoB + %d];" % (ia, ib))
+                W("      const T pab = PA[mq + (size_t)(aoA + %d) * nbf + aoB + %d];" % (ia, ib))
                 W("      T jab = 0;")
                 for ic in range(nC):
                     for id_ in range(nD):
@@ -1014,15 +1025,15 @@ class Gen:
             zero("Kac", nC)
             zero("Kad", nD)
             for id_ in range(nD):
-                W("      const T Pad_%d = PA[(size_t)(aoA + %d) * nbf + aoD + %d];" % (id_, ia, id_))
+                W("      const T Pad_%d = PA[mq + (size_t)(aoA + %d) * nbf + aoD + %d];" % (id_, ia, id_))
             for ic in range(nC):
-                W("      const T Pac_%d = PA[(size_t)(aoA + %d) * nbf + aoC + %d];" % (ic, ia, ic))
+                W("      const T Pac_%d = PA[mq + (size_t)(aoA + %d) * nbf + aoC + %d];" % (ic, ia, ic))
             for ib in range(nB):
                 W("      {")
                 for id_ in range(nD):
-                    W("      const T Pbd_%d = PA[(size_t)(aoB + %d) * nbf + aoD + %d];" % (id_, ib, id_))
+                    W("      const T Pbd_%d = PA[mq + (size_t)(aoB + %d) * nbf + aoD + %d];" % (id_, ib, id_))
                 for ic in range(nC):
-                    W("      const T Pbc_%d = PA[(size_t)(aoB + %d) * nbf + aoC + %d];" % (ic, ib, ic))
+                    W("      const T Pbc_%d = PA[mq + (size_t)(aoB + %d) * nbf + aoC + %d];" % (ic, ib, ic))
                 for ic in range(nC):
                     for id_ in range(nD):
                         W("      { const T v = bp[%d]; Kac_%d = fma(v, Pbd_%d, Kac_%d); Kad_%d = fma(v, Pbc_%d, Kad_%d);"

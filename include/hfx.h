@@ -114,9 +114,47 @@ int hfx_fock_diag_device(hfx_ctx* ctx, const double* dP, double* dF, double thr,
 int hfx_fock_initial_diag_device(hfx_ctx* ctx, const double* dP, double* dF, double thr, void* stream);
 
 /* The context's HIP stream (a hipStream_t), for callers that enqueue their own
- * work in order with the builds, and its number of basis functions. */
+ * work in order with the builds, its number of basis functions (for a batch
+ * context: the row stride of its stacked matrices, the largest molecule's
+ * basis size) and its number of molecules. */
 void* hfx_ctx_stream(hfx_ctx* ctx);
 int hfx_ctx_nb(hfx_ctx* ctx);
+int hfx_ctx_nmol(hfx_ctx* ctx);
+
+/* ---- Batches: many molecules, one launch sequence ----------------------
+ * No counterpart in the reference, which builds one molecule per process
+ * (fock_build_non_diag.f90:33-57 distributes one molecule's quartets over
+ * OpenMP threads).  A batch context holds nmol molecules: each class is
+ * screened and evaluated for all of them in one launch of its kernels, over
+ * one quartet list that never pairs shells of different molecules.  Its
+ * matrices are stacked: molecule m's nb x nb block (nb = hfx_ctx_nb(ctx),
+ * the largest molecule's basis size; a smaller molecule uses the top-left
+ * nb_m x nb_m part and leaves the rest zero) starts at element m * nb * nb.
+ * Every entry point above takes such stacked arrays on a batch context (the
+ * host-array builds copy nmol * nb * nb doubles each way); hfx_ctx_onee needs
+ * a single-molecule context. */
+int hfx_batch_create(const hfx_mol_desc* descs, int nmol, hfx_ctx** out);
+/* Schwarz tables of molecule m (0-based), as hfx_get_schwarz. */
+int hfx_batch_get_schwarz(hfx_ctx* ctx, int m, int t, double* schw, int* order, int* last);
+
+#define HFX_BUILD_ZERO_F 1      /* hfx_batch_fock_device: F = 0 before the build */
+#define HFX_BUILD_GRAPH 2       /* ... : capture the build in a HIP graph on its first call with these
+                                   arguments and replay it on later calls (no host synchronisation, no
+                                   count read-back; the pointed-to values may change between calls) */
+/* One SCF iteration's two-electron build for every molecule: F += diagonal-
+ * pair part at thr_diag[m] (fock_build_diag) + off-diagonal part at thr_nd[m]
+ * (fock_build_non_diag), mode HFX_JK or HFX_J_ONLY.  thr_diag, thr_nd and
+ * split (FP32 split per molecule, or NULL for the context's
+ * HFX_OPT_MIXED_SPLIT) are device arrays of nmol doubles; a molecule with an
+ * infinite threshold adds nothing.  Needs the Schwarz tables
+ * (hfx_fock_initial_diag*). */
+int hfx_batch_fock_device(hfx_ctx* ctx, const double* dP, double* dF, const double* thr_diag, const double* thr_nd,
+                          const double* split, int mode, int flags, void* stream);
+/* One-electron matrices S, T, V of every molecule (stacked host arrays,
+ * nmol * nb * nb each), molecule m with natom[m] point charges; charges and
+ * coordinates (bohr) of all molecules concatenated.  hfx_ctx_onee for a batch. */
+int hfx_batch_onee(hfx_ctx* ctx, const int* natom, const double* charges, const double* coords, double* S, double* T,
+                   double* V);
 
 /* Statistics of the most recent build (blocks until it has finished). */
 int hfx_get_stats(hfx_ctx* ctx, hfx_stats* out);
@@ -209,6 +247,34 @@ void hfx_scf_defaults(hfx_scf_options* opt);
 int hfx_scf_run(hfx_ctx* ctx, const double* S, const double* H, int nocc, double unuc, const double* P0,
                 const hfx_scf_options* opt, hfx_scf_result* res, double* P, double* F, double* eigs,
                 double* history);
+
+/* Direct RHF for every molecule of a batch context at once (the driver loop
+ * of direct_hartreefock.f90:98-223, as hfx_scf_run, for all of them): every
+ * molecule's S, H, S^-1/2, P, F, DIIS history and orbitals stay in device
+ * memory; per iteration one shared two-electron build (hfx_batch_fock_device,
+ * captured in a HIP graph when use_graphs != 0, as is the sequence of
+ * products and reductions after it), batched rocBLAS products, one batched
+ * rocSOLVER eigensolve, and one small read-back of every molecule's energy,
+ * max|error|, max|dP| and DIIS inner products for the host's per-molecule
+ * decisions.  Full builds use the context's HFX_OPT_MIXED_SPLIT, incremental
+ * builds run in FP64.  A molecule that has converged (or reached maxiter)
+ * adds no work to later shared builds and keeps its density.
+ * S, H, P0 (or NULL: core-Hamiltonian guess) are stacked host arrays as the
+ * context's; nocc and unuc one per molecule; opt->levelshift must be 0.
+ * No counterpart in the reference (one molecule per process). */
+typedef struct hfx_batch_scf hfx_batch_scf;
+int hfx_batch_scf_create(hfx_ctx* ctx, const double* S, const double* H, const int* nocc, const double* unuc,
+                         const double* P0, const hfx_scf_options* opt, int use_graphs, hfx_batch_scf** out);
+/* One iteration of every molecule still iterating; per molecule status
+ * (0 iterating, 1 converged, 2 maxiter reached without converging, < 0 an
+ * HFX_E_* failure of that molecule), energy (etot of its last iteration) and
+ * iterations (each may be NULL).  Returns the number still iterating, or a
+ * negative HFX_E_* code. */
+int hfx_batch_scf_step(hfx_batch_scf* b, int* status, double* energy, int* iterations);
+/* Molecule m's density of its last build, that build's Fock matrix and its
+ * orbital energies (nb_m * nb_m, nb_m * nb_m, nb_m; each may be NULL). */
+int hfx_batch_scf_result(hfx_batch_scf* b, int m, double* P, double* F, double* eigs);
+void hfx_batch_scf_destroy(hfx_batch_scf* b);
 
 const char* hfx_version(void);
 

@@ -40,6 +40,7 @@ def build(jobs=8):
     gen_onee.main(GEN)
     os.makedirs(OBJ, exist_ok=True)
     srcs = [os.path.join(CSRC, "hfx_kernels.hip"), os.path.join(CSRC, "hfx_host.hip"), os.path.join(CSRC, "hfx_scf.hip"),
+            os.path.join(CSRC, "hfx_batch.hip"),
             os.path.join(HERE, "emu_smem.cpp"), os.path.join(CSRC, "hfx_onee.cpp"),
             os.path.join(CSRC, "hfx_input.cpp")]
     srcs += sorted(os.path.join(GEN, f) for f in os.listdir(GEN) if f.endswith(".hip"))

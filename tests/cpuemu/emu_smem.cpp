@@ -1,5 +1,28 @@
-// TEST INFRASTRUCTURE: backing store of the emulated LDS (see hip/hip_runtime.h).
+// TEST INFRASTRUCTURE: backing store of the emulated LDS, the emulated block
+// index, and the argument size of type-erased launches (see hip/hip_runtime.h).
 #include "hip/hip_runtime.h"
+
+#include <unordered_set>
+
+#include "eri_dispatch.h"
+#include "hfx_kernels.h"
+
+thread_local hfx_emu_idx blockIdx = {0, 0, 0};
+thread_local hfx_emu_idx gridDim = {1, 1, 1};
+
 namespace hfx {
 alignas(64) char hfx_smem[HFX_EMU_SMEM_BYTES];
+}
+
+size_t hfx_emu_arg_size(const void* fn) {
+  static const std::unordered_set<const void*>* dig = [] {
+    auto* s = new std::unordered_set<const void*>;
+    for (const auto& k : hfx::kClassKernels)
+      for (const void* f : {k.dig, k.digj, k.digf})
+        if (f) s->insert(f);
+    return s;
+  }();
+  static_assert(sizeof(hfx::DigestArgs) <= sizeof(hfx_emu_argbuf) && sizeof(hfx::ClassArgs) <= sizeof(hfx_emu_argbuf),
+                "argument buffer too small");
+  return dig->count(fn) ? sizeof(hfx::DigestArgs) : sizeof(hfx::ClassArgs);
 }

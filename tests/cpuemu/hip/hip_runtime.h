@@ -3,8 +3,10 @@
 // kernels and generated class kernels) can be compiled with g++ into
 // tests/cpuemu/libhfx_cpuemu.so and checked on a machine without a GPU.
 //
-// Every launch runs as one workgroup of one thread (HFX_WAVE = 1): the
-// kernels are grid-stride loops, so one thread visits every work item.
+// Every launch runs as one workgroup of one thread (HFX_WAVE = 1) per
+// blockIdx.y (the kernels are grid-stride loops in x, so one thread visits
+// every work item).  Work launched while a stream is being captured is
+// recorded instead and run again by hipGraphLaunch, as a HIP graph would.
 // This library is loaded only by tests; the product (hfcode_amd/fock.py)
 // loads the gfx950 build libhfx.so and nothing else.
 #pragma once
@@ -13,6 +15,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #ifndef HFX_WAVE
@@ -45,9 +48,9 @@ struct hfx_emu_idx {
   unsigned x, y, z;
 };
 static const hfx_emu_idx threadIdx = {0, 0, 0};
-static const hfx_emu_idx blockIdx = {0, 0, 0};
 static const hfx_emu_idx blockDim = {1, 1, 1};
-static const hfx_emu_idx gridDim = {1, 1, 1};
+extern thread_local hfx_emu_idx blockIdx;   // x = 0; y runs over the launch's grid.y
+extern thread_local hfx_emu_idx gridDim;    // x = 1
 
 inline void __syncthreads() {}
 inline int __syncthreads_count(int p) { return p ? 1 : 0; }
@@ -111,15 +114,82 @@ inline hipError_t hipFree(void* p) {
   std::free(p);
   return hipSuccess;
 }
+inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) { return hipMalloc(p, n); }
+inline hipError_t hipHostFree(void* p) { return hipFree(p); }
+// stream capture: the recorded operations of the graph being captured (per thread)
+typedef std::vector<std::function<void()>> hfx_emu_ops;
+inline hfx_emu_ops*& hfx_emu_capture() {
+  static thread_local hfx_emu_ops* cur = nullptr;
+  return cur;
+}
+// runs f once per blockIdx.y of the grid, or records that for a graph
+inline void hfx_emu_launch(unsigned gy, std::function<void()> f) {
+  auto run = [gy, f]() {
+    const hfx_emu_idx b = blockIdx, g = gridDim;
+    gridDim = hfx_emu_idx{1, gy, 1};
+    for (unsigned y = 0; y < gy; ++y) {
+      blockIdx = hfx_emu_idx{0, y, 0};
+      f();
+    }
+    blockIdx = b;
+    gridDim = g;
+  };
+  if (hfx_emu_ops* c = hfx_emu_capture()) c->push_back(run);
+  else run();
+}
 inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
   std::memmove(d, s, n);
   return hipSuccess;
 }
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) {
+  if (hfx_emu_ops* c = hfx_emu_capture()) {
+    c->push_back([=]() { std::memmove(d, s, n); });
+    return hipSuccess;
+  }
   return hipMemcpy(d, s, n, k);
 }
 inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) {
+  if (hfx_emu_ops* c = hfx_emu_capture()) {
+    c->push_back([=]() { std::memset(d, v, n); });
+    return hipSuccess;
+  }
   std::memset(d, v, n);
+  return hipSuccess;
+}
+// graphs
+struct hfx_emu_graph {
+  hfx_emu_ops ops;
+};
+typedef hfx_emu_graph* hipGraph_t;
+typedef hfx_emu_graph* hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+  if (hfx_emu_capture()) return 4;
+  hfx_emu_capture() = new hfx_emu_ops;
+  return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+  hfx_emu_ops* c = hfx_emu_capture();
+  if (!c) return 4;
+  *g = new hfx_emu_graph{std::move(*c)};
+  delete c;
+  hfx_emu_capture() = nullptr;
+  return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, char*, size_t) {
+  *e = new hfx_emu_graph(*g);
+  return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+  for (auto& f : e->ops) f();
+  return hipSuccess;
+}
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
+  delete e;
+  return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) {
+  delete g;
   return hipSuccess;
 }
 inline hipError_t hipGetDevice(int* d) {
@@ -149,21 +219,32 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
   *ms = 0.f;
   return hipSuccess;
 }
-#define hipLaunchKernelGGL(k, g, b, shm, st, ...) k(__VA_ARGS__)
+// the arguments are evaluated (and copied) at the launch, as HIP does
+template <typename K, typename... A>
+inline void hfx_emu_kernel(dim3 g, K k, A... a) {
+  hfx_emu_launch(g.y, [=]() { k(a...); });
+}
+#define hipLaunchKernelGGL(k, g, b, shm, st, ...) hfx_emu_kernel(dim3(g), k, __VA_ARGS__)
 inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t) {
   *n = 1;
   return hipSuccess;
 }
 
-namespace hfx {
-struct ClassArgs;
-}
-// Class kernels are launched through a type-erased pointer; all of them take
-// one ClassArgs by value.
-template <typename A = hfx::ClassArgs>
-inline hipError_t hipLaunchKernel(const void* fn, dim3, dim3, void** args, size_t shm, hipStream_t) {
+// Class and digestion kernels are launched through a type-erased pointer with
+// one argument struct by value (ClassArgs or DigestArgs).  The stand-in copies
+// that struct (its size from hfx_emu_arg_size, which knows the digestion
+// kernels) into a larger zeroed buffer and passes the buffer by value: both
+// are aggregates passed in memory, so the kernel reads its own struct.
+struct hfx_emu_argbuf {
+  alignas(16) char b[512];
+};
+size_t hfx_emu_arg_size(const void* fn);
+inline hipError_t hipLaunchKernel(const void* fn, dim3 g, dim3, void** args, size_t shm, hipStream_t) {
   if (shm > HFX_EMU_SMEM_BYTES) return 3;
-  typedef void (*K)(const A);
-  ((K)fn)(*(const A*)args[0]);
+  hfx_emu_argbuf buf;
+  std::memset(buf.b, 0, sizeof(buf.b));
+  std::memcpy(buf.b, args[0], hfx_emu_arg_size(fn));
+  typedef void (*K)(hfx_emu_argbuf);
+  hfx_emu_launch(g.y, [fn, buf]() { ((K)fn)(buf); });
   return hipSuccess;
 }

@@ -2,9 +2,13 @@
 // libhfx's SCF driver calls (column-major dgemm), for the CPU emulation build
 // of tests/cpuemu.  Loaded by tests only.
 #pragma once
+#include <cstdint>
 #include <vector>
 
+#include "../hip/hip_runtime.h"
+
 typedef int rocblas_int;
+typedef int64_t rocblas_stride;
 typedef enum { rocblas_status_success = 0, rocblas_status_invalid_value = 3 } rocblas_status;
 typedef enum { rocblas_operation_none = 111, rocblas_operation_transpose = 112 } rocblas_operation;
 typedef enum { rocblas_fill_upper = 121, rocblas_fill_lower = 122 } rocblas_fill;
@@ -25,7 +29,7 @@ inline rocblas_status rocblas_destroy_handle(rocblas_handle h) {
 inline rocblas_status rocblas_set_stream(rocblas_handle, void*) { return rocblas_status_success; }
 
 // C = alpha op(A) op(B) + beta C, column-major
-inline rocblas_status rocblas_dgemm(rocblas_handle, rocblas_operation ta, rocblas_operation tb, rocblas_int m,
+inline rocblas_status hfx_emu_dgemm(rocblas_handle, rocblas_operation ta, rocblas_operation tb, rocblas_int m,
                                     rocblas_int n, rocblas_int k, const double* alpha, const double* A,
                                     rocblas_int lda, const double* B, rocblas_int ldb, const double* beta, double* C,
                                     rocblas_int ldc) {
@@ -42,5 +46,26 @@ inline rocblas_status rocblas_dgemm(rocblas_handle, rocblas_operation ta, rocbla
     }
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < m; ++i) C[(size_t)j * ldc + i] = out[(size_t)j * m + i];
+  return rocblas_status_success;
+}
+
+// recorded like a kernel while a stream is captured (see hip/hip_runtime.h)
+inline rocblas_status rocblas_dgemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, rocblas_int m,
+                                    rocblas_int n, rocblas_int k, const double* alpha, const double* A,
+                                    rocblas_int lda, const double* B, rocblas_int ldb, const double* beta, double* C,
+                                    rocblas_int ldc) {
+  const double al = *alpha, be = *beta;
+  hfx_emu_launch(1, [=]() { hfx_emu_dgemm(h, ta, tb, m, n, k, &al, A, lda, B, ldb, &be, C, ldc); });
+  return rocblas_status_success;
+}
+
+inline rocblas_status rocblas_dgemm_strided_batched(rocblas_handle h, rocblas_operation ta, rocblas_operation tb,
+                                                    rocblas_int m, rocblas_int n, rocblas_int k, const double* alpha,
+                                                    const double* A, rocblas_int lda, rocblas_stride sA,
+                                                    const double* B, rocblas_int ldb, rocblas_stride sB,
+                                                    const double* beta, double* C, rocblas_int ldc, rocblas_stride sC,
+                                                    rocblas_int batch) {
+  for (rocblas_int b = 0; b < batch; ++b)
+    rocblas_dgemm(h, ta, tb, m, n, k, alpha, A + b * sA, lda, B + b * sB, ldb, beta, C + b * sC, ldc);
   return rocblas_status_success;
 }

@@ -10,7 +10,7 @@
 
 #include "../rocblas/rocblas.h"
 
-inline rocblas_status rocsolver_dsyevd(rocblas_handle, rocblas_evect evect, rocblas_fill uplo, rocblas_int n,
+inline rocblas_status hfx_emu_dsyevd(rocblas_handle, rocblas_evect evect, rocblas_fill uplo, rocblas_int n,
                                        double* A, rocblas_int lda, double* D, double* E, rocblas_int* info) {
   (void)E;
   std::vector<double> a((size_t)n * n), v((size_t)n * n, 0.0);
@@ -60,5 +60,20 @@ inline rocblas_status rocsolver_dsyevd(rocblas_handle, rocblas_evect evect, rocb
       for (int i = 0; i < n; ++i) A[(size_t)j * lda + i] = v[(size_t)idx[j] * n + i];
   }
   *info = 0;
+  return rocblas_status_success;
+}
+
+inline rocblas_status rocsolver_dsyevd(rocblas_handle h, rocblas_evect evect, rocblas_fill uplo, rocblas_int n,
+                                       double* A, rocblas_int lda, double* D, double* E, rocblas_int* info) {
+  hfx_emu_launch(1, [=]() { hfx_emu_dsyevd(h, evect, uplo, n, A, lda, D, E, info); });
+  return rocblas_status_success;
+}
+
+inline rocblas_status rocsolver_dsyevd_strided_batched(rocblas_handle h, rocblas_evect evect, rocblas_fill uplo,
+                                                       rocblas_int n, double* A, rocblas_int lda, rocblas_stride sA,
+                                                       double* D, rocblas_stride sD, double* E, rocblas_stride sE,
+                                                       rocblas_int* info, rocblas_int batch) {
+  for (rocblas_int b = 0; b < batch; ++b)
+    rocsolver_dsyevd(h, evect, uplo, n, A + b * sA, lda, D + b * sD, E + b * sE, info + b);
   return rocblas_status_success;
 }
