@@ -444,6 +444,8 @@ int upload_active(hfx_ctx* c) {
       cap += cp.same ? nb * (nb - 1) / 2 : nb * nk;
     }
     cp.cap = cap;
+    // quartet positions and counts are 32-bit on the device
+    if (cp.cap > (long long)INT32_MAX - 4 * HFX_NT) return HFX_E_UNSUPPORTED;
     if (cp.cap > 0 && dalloc(&cp.list64, (size_t)cp.cap)) return HFX_E_NOMEM;
     need_masks = std::max(need_masks, (size_t)c->act_len[cp.t1] * (size_t)c->nchunk_max[cp.t2] * 2);
   }
@@ -576,6 +578,13 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     int r = ensure_side_streams(c);
     if (r) return r;
   }
+  // every chunk's blocks and partial maxima stay inside its half (or the
+  // whole buffer) of the integral buffer and of vmax
+  if (chunk < HFX_NT || chunk * ne > (pipelined ? half_e : c->eblk_doubles) ||
+      chunk > (pipelined ? c->vmax_rows / 2 : c->vmax_rows) || k.ncombo > kMaxCombo || k.ncombo32 > kMaxCombo) {
+    fprintf(stderr, "hfx: chunk of %lld quartets does not fit the integral buffer\n", chunk);
+    return HFX_E_STATE;
+  }
   const int K = prec == 1 ? 5 : 8;
   // LDS rows of X for the digestion (off for the diagonal builds: every bra once)
   const size_t rows_shm = (size_t)(k.dims[0] + k.dims[1]) * c->nb * sizeof(double);
@@ -673,11 +682,12 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     void* gargs[] = {&g};
     // FP32 quartets: blocks stored as floats and contracted with the FP32 copy
     // of P; every quartet's sums are added to X in FP64
-    const void* dfn = prec == 1 ? k.digf : (jonly ? k.digj : k.dig);
+    const bool multi = c->nmol > 1;   // molecule-aware form only for batch contexts
+    const void* dfn = prec == 1 ? (multi ? k.digfm : k.digf) : (jonly ? (multi ? k.digjm : k.digj) : (multi ? k.digm : k.dig));
     // with LDS rows: one workgroup per resident slot (registers and LDS, as
     // the runtime reports them; at most four per CU), each a contiguous range
     // of steps
-    int& occ = c->dig_occ[prec == 1 ? 2 : (jonly ? 1 : 0)][cls];
+    int& occ = c->dig_occ[prec == 1 ? 2 : (jonly ? 1 : 0)][cls];   // one context: one of the two forms
     if (rows && occ == 0) {
       int nblk = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, dfn, HFX_NT, dig_shm) != hipSuccess || nblk < 1) {

@@ -180,11 +180,17 @@ struct Solver {
   double* ring_r[kMaxDiis] = {};
   double* ring_f[kMaxDiis] = {};
   double *part = nullptr, *scal = nullptr;
+  double* par = nullptr;          // build parameters in device memory: [thr_full, thr_diag, thr_nd, split, 0]
+  double* hpar = nullptr;         // their pinned host copy
   rocblas_int* info = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
 
   ~Solver() {
     if (st) hipStreamSynchronize(st);   // nothing queued may still use the buffers
+    if (hpar) {
+      hfx::DeviceSection ds;
+      hipHostFree(hpar);
+    }
     for (double* b : bufs) hfx::cached_free(b);
     if (info) hfx::cached_free(info);
     if (hb) give_handle(hb);
@@ -294,6 +300,12 @@ int run(Solver& s, const double* S, const double* H, int nocc, double unuc, cons
       !(s.scal = s.alloc(kNumSlots)))
     return HFX_E_NOMEM;
   if (!(s.info = static_cast<rocblas_int*>(hfx::cached_alloc(sizeof(rocblas_int))))) return HFX_E_NOMEM;
+  if (!(s.par = s.alloc(8))) return HFX_E_NOMEM;
+  {
+    hfx::DeviceSection ds;
+    if (hipHostMalloc((void**)&s.hpar, 8 * sizeof(double), 0) != hipSuccess) return HFX_E_NOMEM;
+  }
+  const double split = hfx::ctx_split(s.ctx);
   SCFHIP(hipEventCreate(&s.ev[0]));
   SCFHIP(hipEventCreate(&s.ev[1]));
   SCFHIP(hipMemcpyAsync(s.S, S, bytes, hipMemcpyHostToDevice, s.st));
@@ -343,11 +355,26 @@ int run(Solver& s, const double* S, const double* H, int nocc, double unuc, cons
   for (it = 1; it <= o.maxiter; ++it) {
     const double eeold = eelec;
     SCFHIP(hipEventRecord(s.ev[0], s.st));
+    // Builds after the first iteration replay HIP graphs (hfx_batch_fock_device
+    // with HFX_BUILD_GRAPH): thresholds and split are read from device memory,
+    // so one graph serves every full build and one every incremental build.
+    // The pinned parameters are rewritten only after the previous iteration's
+    // copy of them has run (the stream is synchronised once per iteration).
+    s.hpar[0] = thresh;                  // full build: diag and non-diag
+    s.hpar[1] = thresh;                  // incremental: diag ...
+    s.hpar[2] = thresh * o.fdifffac;     // ... and non-diag
+    s.hpar[3] = split;                   // full builds: the context's FP32 split
+    s.hpar[4] = 0.0;                     // incremental builds: FP64
+    SCFHIP(hipMemcpyAsync(s.par, s.hpar, 5 * sizeof(double), hipMemcpyHostToDevice, s.st));
     if (fullfock) {  // F = H + G(P)
       SCFHIP(hipMemcpyAsync(s.F, s.H, bytes, hipMemcpyDeviceToDevice, s.st));
-      if (it == 1) SCFBUILD(hfx_fock_initial_diag_device(s.ctx, s.P, s.F, thresh, nullptr));
-      else SCFBUILD(hfx_fock_diag_device(s.ctx, s.P, s.F, thresh, nullptr));
-      SCFBUILD(hfx_fock_nd_device(s.ctx, s.P, s.F, thresh, o.mode, nullptr));
+      if (it == 1) {
+        SCFBUILD(hfx_fock_initial_diag_device(s.ctx, s.P, s.F, thresh, nullptr));
+        SCFBUILD(hfx_fock_nd_device(s.ctx, s.P, s.F, thresh, o.mode, nullptr));
+      } else {
+        SCFBUILD(hfx_batch_fock_device(s.ctx, s.P, s.F, s.par, s.par, split > 0.0 ? s.par + 3 : nullptr, o.mode,
+                                       o.use_graphs ? HFX_BUILD_GRAPH : 0, nullptr));
+      }
       if (it > 1) fullfock = false;
       nfull = 0;
     } else {         // F = F_old + G(P - P_old), non-diagonal part at thresh * fdifffac
@@ -355,9 +382,8 @@ int run(Solver& s, const double* S, const double* H, int nocc, double unuc, cons
       // the split is an absolute bound, and near convergence almost every
       // quartet of dP would fall below it.
       hipLaunchKernelGGL(k_axpy, dim3(s.grid()), dim3(256), 0, s.st, s.P, s.Pold, -1.0, s.dP, s.n2);
-      SCFHIP(hipMemsetAsync(s.dF, 0, bytes, s.st));
-      SCFBUILD(hfx_fock_diag_device(s.ctx, s.dP, s.dF, thresh, nullptr));
-      SCFBUILD(hfx::fock_nd_device_split(s.ctx, s.dP, s.dF, thresh * o.fdifffac, o.mode, nullptr, 0.0));
+      SCFBUILD(hfx_batch_fock_device(s.ctx, s.dP, s.dF, s.par + 1, s.par + 2, split > 0.0 ? s.par + 4 : nullptr,
+                                     o.mode, HFX_BUILD_ZERO_F | (o.use_graphs ? HFX_BUILD_GRAPH : 0), nullptr));
       hipLaunchKernelGGL(k_axpy, dim3(s.grid()), dim3(256), 0, s.st, s.Fold, s.dF, 1.0, s.F, s.n2);
       if (++nfull == 15) fullfock = true;
     }
@@ -486,6 +512,7 @@ void hfx_scf_defaults(hfx_scf_options* o) {
   o->diis_start = 0.3;
   o->levelshift = 0.0;
   o->mode = HFX_JK;
+  o->use_graphs = 1;
 }
 
 int hfx_scf_run(hfx_ctx* ctx, const double* S, const double* H, int nocc, double unuc, const double* P0,

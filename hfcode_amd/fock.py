@@ -71,7 +71,7 @@ class ScfOptions(ctypes.Structure):
     _fields_ = [("maxiter", ctypes.c_int), ("ndiis", ctypes.c_int), ("conv_e", ctypes.c_double),
                 ("conv_p", ctypes.c_double), ("lowthresh", ctypes.c_double), ("intthresh", ctypes.c_double),
                 ("fdifffac", ctypes.c_double), ("diis_start", ctypes.c_double), ("levelshift", ctypes.c_double),
-                ("mode", ctypes.c_int)]
+                ("mode", ctypes.c_int), ("use_graphs", ctypes.c_int)]
 
 
 class ScfResult(ctypes.Structure):
@@ -398,7 +398,7 @@ class FockBuilder:
                                         ctypes.c_void_p(stream or 0)))
 
     def scf(self, S, H, nocc, unuc, P0=None, levelshift=None, maxiter=25, ndiis=8, conv_e=5e-8, conv_p=1e-5,
-            lowthresh=5e-7, intthresh=1e-10, fdifffac=0.1, diis_start=0.3, mode="jk"):
+            lowthresh=5e-7, intthresh=1e-10, fdifffac=0.1, diis_start=0.3, mode="jk", use_graphs=True):
         """Direct RHF with every step on the GPU (hfx_scf_run): the driver of
         direct_hartreefock.f90:98-223 as hfcode_amd.scf.rhf restates it.
         Returns an object with energy, P, F, eigs, iterations, converged,
@@ -416,6 +416,7 @@ class FockBuilder:
         o.diis_start = min(float(diis_start), 1e300)
         o.levelshift = float(levelshift or 0.0)
         o.mode = HFX_J_ONLY if mode == "j" else HFX_JK
+        o.use_graphs = int(bool(use_graphs))
         r = ScfResult()
         P, F, eigs = np.zeros((nb, nb)), np.zeros((nb, nb)), np.zeros(nb)
         hist = np.zeros((maxiter, 4))

@@ -230,6 +230,7 @@ typedef struct hfx_scf_options {
   double diis_start;    /* DIIS starts when max|resid| < this (0.3) */
   double levelshift;    /* lshft; 0 = no level shift (levshft false) */
   int mode;             /* HFX_JK */
+  int use_graphs;       /* 1: builds after the first iteration replay HIP graphs (hfx_batch_fock_device) */
 } hfx_scf_options;
 typedef struct hfx_scf_result {
   double energy;        /* etot of the last iteration */
