@@ -182,6 +182,42 @@ void dev_free(void* p) {
 
 }  // namespace
 
+// Device buffers of hfx_ctx_onee_noalloc: a fixed array in every GPU's copy
+// of the code object, held until the process ends.  Its host-side name is
+// not a device address; each device's address is taken once
+// (scratch_address) and used everywhere host code passes the array on.
+constexpr size_t kScratchBytes = (size_t)48 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;   // per device, null until taken
+std::vector<char> g_scratch_done;
+}  // namespace
+
+// The scratch array's address on the current device (taken on the first
+// call per device, with that device current); null if HIP refuses it.
+static void* scratch_address(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0) return nullptr;
+  if ((size_t)dev >= g_scratch_ptr.size()) {
+    g_scratch_ptr.resize(dev + 1, nullptr);
+    g_scratch_done.resize(dev + 1, 0);
+  }
+  if (!g_scratch_done[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      fprintf(stderr, "hfx: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+      p = nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+    g_scratch_done[dev] = 1;
+  }
+  return g_scratch_ptr[dev];
+}
+
 std::shared_mutex& hfx::capture_mutex() {
   static std::shared_mutex* m = new std::shared_mutex;   // lives until the process ends
   return *m;
@@ -272,6 +308,7 @@ struct hfx_ctx {
   bool capturing = false;         // a HIP graph is being captured from the context's stream
   std::vector<CachedGraph> graphs;  // captured builds; dropped whenever a buffer they use is replaced
   bool graphs_off = false;          // a capture failed: later builds run uncaptured
+  void* scratch = nullptr;          // address of the scratch array on the context's device
   std::vector<PairRec> pairs;
   std::vector<PrimRec> prims;
   std::vector<ClassPlan> classes;
@@ -1182,6 +1219,7 @@ int hfx_batch_create(const hfx_mol_desc* descs, int nmol, hfx_ctx** out) {
   if (hipGetDeviceProperties(&prop, c->dev) != hipSuccess) return fail(HFX_E_HIP);
   c->ncu = prop.multiProcessorCount;
   c->max_shm = prop.sharedMemPerBlock;
+  c->scratch = scratch_address(c->dev);   // set-up of hfx_ctx_onee_noalloc, once per device
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(HFX_E_HIP);
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(HFX_E_HIP);
   std::vector<ShellRec> sr(c->nsh);
@@ -1310,116 +1348,175 @@ int hfx_fock_diag(hfx_ctx* c, const double* P, double* F, int nb, double thr) {
   return fetch_fock_host(c, F, nb);
 }
 
+}  // extern "C"
+
+namespace {
+
+// One-electron set-up of a context: the work items per (molecule, la, lb)
+// of the shell-pair parts and the shell tables the kernels read, built on
+// the host; where they and the temporaries live is the caller's (library
+// allocations, or the fixed scratch array of hfx_ctx_onee_noalloc).
+struct OneeTables {
+  std::vector<int4> items;
+  std::vector<int> count;                    // per (molecule, la, lb)
+  std::vector<int> ty, np, ao;
+  std::vector<double> xyz, z, cc, cl;
+};
+struct OneeDev {
+  int4* items;
+  int *ty, *np, *ao;
+  double *xyz, *z, *cc, *cl;
+  double *q, *cxyz;                          // charges, coordinates of every molecule's atoms
+  double *S, *T, *V;                         // nmol * nb * nb each
+};
+constexpr int kOneeMaxPrim = 16;
+
+void onee_tables(const hfx_ctx* c, OneeTables& t) {
+  auto nparts = [](int ty) { return ty == 12 ? 2 : 1; };
+  auto part_l = [](int ty, int p) { return ty == 1 ? 0 : ty == 2 ? 1 : ty == 3 ? 2 : p; };
+  t.count.assign((size_t)c->nmol * 9, 0);
+  t.items.clear();
+  for (int m = 0; m < c->nmol; ++m) {
+    const MolPart& mp = c->mols[m];
+    std::vector<int4> items[3][3];
+    for (int i = mp.sh0; i < mp.sh0 + mp.nsh; ++i)
+      for (int j = mp.sh0; j <= i; ++j)
+        for (int pa = 0; pa < nparts(c->shells[i].type); ++pa)
+          for (int pb = 0; pb < nparts(c->shells[j].type); ++pb)
+            items[part_l(c->shells[i].type, pa)][part_l(c->shells[j].type, pb)].push_back(make_int4(i, j, pa, pb));
+    for (int la = 0; la < 3; ++la)
+      for (int lb = 0; lb < 3; ++lb) {
+        t.count[(size_t)m * 9 + la * 3 + lb] = (int)items[la][lb].size();
+        t.items.insert(t.items.end(), items[la][lb].begin(), items[la][lb].end());
+      }
+  }
+  const int nsh = c->nsh, mp = kOneeMaxPrim;
+  t.ty.resize(nsh); t.np.resize(nsh); t.ao.resize(nsh);
+  t.xyz.resize(3 * (size_t)nsh);
+  t.z.assign((size_t)mp * nsh, 0.0); t.cc.assign((size_t)mp * nsh, 0.0); t.cl.assign((size_t)mp * nsh, 0.0);
+  for (int i = 0; i < nsh; ++i) {
+    const Shell& sh = c->shells[i];
+    t.ty[i] = sh.type;
+    t.np[i] = sh.nprim;
+    t.ao[i] = sh.ao;
+    for (int k = 0; k < 3; ++k) t.xyz[3 * i + k] = sh.x[k];
+    for (int k = 0; k < mp; ++k) {
+      t.z[(size_t)mp * i + k] = sh.zeta[k];
+      t.cc[(size_t)mp * i + k] = sh.djk[k];
+      t.cl[(size_t)mp * i + k] = sh.djkl[k];
+    }
+  }
+}
+
+int onee_upload_tables(hfx_ctx* c, const OneeTables& t, const OneeDev& d) {
+  const int nsh = c->nsh;
+  HIPOK(hipMemcpyAsync(d.items, t.items.data(), t.items.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+  HIPOK(hipMemcpyAsync(d.ty, t.ty.data(), nsh * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPOK(hipMemcpyAsync(d.np, t.np.data(), nsh * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPOK(hipMemcpyAsync(d.ao, t.ao.data(), nsh * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPOK(hipMemcpyAsync(d.xyz, t.xyz.data(), t.xyz.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPOK(hipMemcpyAsync(d.z, t.z.data(), t.z.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPOK(hipMemcpyAsync(d.cc, t.cc.data(), t.cc.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPOK(hipMemcpyAsync(d.cl, t.cl.data(), t.cl.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return HFX_OK;
+}
+
+// The launches: charges and coordinates of ntot atoms uploaded, S, T, V
+// cleared (a fill kernel), one kernel pair per (molecule, la, lb), the
+// matrices copied back to the host; waits for all of it.
+int onee_run(hfx_ctx* c, const OneeDev& d, const std::vector<int>& count, const int* natom, int ntot,
+             const double* charges, const double* coords, double* S, double* T, double* V) {
+  const int nb = c->nb;
+  const size_t nmat = (size_t)nb * nb * c->nmol;
+  if (ntot > 0) {
+    HIPOK(hipMemcpyAsync(d.q, charges, ntot * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPOK(hipMemcpyAsync(d.cxyz, coords, 3 * (size_t)ntot * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  for (double* x : {d.S, d.T, d.V})   // the padding of smaller molecules' blocks stays zero
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)nmat, 256, 4096)), dim3(256), 0, c->stream, x, nmat, 0.0);
+  OneeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.sh_type = d.ty; a.sh_np = d.np; a.sh_ao = d.ao; a.sh_xyz = d.xyz;
+  a.sh_z = d.z; a.sh_c = d.cc; a.sh_cl = d.cl; a.maxprim = kOneeMaxPrim;
+  a.boys64 = c->d_boys64;
+  a.nb = nb;
+  int off = 0, atom0 = 0;
+  for (int m = 0; m < c->nmol; ++m) {
+    const size_t base = (size_t)m * nb * nb;
+    a.charges = d.q + atom0; a.coords = d.cxyz + 3 * atom0; a.natom = natom[m];
+    a.S = d.S + base; a.T = d.T + base; a.V = d.V + base;
+    atom0 += natom[m];
+    for (int la = 0; la < 3; ++la)
+      for (int lb = 0; lb < 3; ++lb) {
+        const int n = count[(size_t)m * 9 + la * 3 + lb];
+        if (n == 0) continue;
+        a.items = d.items + off;
+        a.nitems = n;
+        off += n;
+        const int grid = grid_for(n, HFX_NT, c->ncu * 8);
+        const size_t shm = (size_t)HFX_BOYS_ROWS * (la + lb + 8) * sizeof(double);
+        hipLaunchKernelGGL(kOneeST[la][lb], dim3(grid), dim3(HFX_NT), 0, c->stream, a);
+        hipLaunchKernelGGL(kOneeV[la][lb], dim3(grid), dim3(HFX_NT), shm, c->stream, a);
+      }
+  }
+  HIPOK(hipGetLastError());
+  HIPOK(hipMemcpyAsync(S, d.S, nmat * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPOK(hipMemcpyAsync(T, d.T, nmat * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPOK(hipMemcpyAsync(V, d.V, nmat * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPOK(hipStreamSynchronize(c->stream));
+  return HFX_OK;
+}
+
+int onee_natoms(const hfx_ctx* c, const int* natom, const double* charges, const double* coords, int* ntot) {
+  *ntot = 0;
+  for (int m = 0; m < c->nmol; ++m) {
+    if (natom[m] < 0) return HFX_E_ARG;
+    *ntot += natom[m];
+  }
+  if (*ntot > 0 && (!charges || !coords)) return HFX_E_ARG;
+  return HFX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 // S, T, V on the device: work items per (molecule, la, lb) of the shell-pair
 // parts, one generated kernel pair per (la, lb); each molecule's matrices in
 // its block of the stacked outputs, its own nuclei for V.  natom[m] atoms of
-// molecule m, their charges and coordinates concatenated.
+// molecule m, their charges and coordinates concatenated.  The tables stay
+// in the context's memory for later calls; the temporaries are allocated
+// per call.
 int hfx_batch_onee(hfx_ctx* c, const int* natom, const double* charges, const double* coords, double* S, double* T,
                    double* V) {
   if (!c || !natom || !S || !T || !V) return HFX_E_ARG;
-  int ntot = 0;
-  for (int m = 0; m < c->nmol; ++m) {
-    if (natom[m] < 0) return HFX_E_ARG;
-    ntot += natom[m];
-  }
-  if (ntot > 0 && (!charges || !coords)) return HFX_E_ARG;
-  const int nsh = c->nsh, nb = c->nb;
+  int ntot = 0, r;
+  if ((r = onee_natoms(c, natom, charges, coords, &ntot))) return r;
+  const int nsh = c->nsh;
   if (!c->d_onee_items) {
-    auto nparts = [](int t) { return t == 12 ? 2 : 1; };
-    auto part_l = [](int t, int p) { return t == 1 ? 0 : t == 2 ? 1 : t == 3 ? 2 : p; };
-    std::vector<int4> all;
-    c->onee_count.assign((size_t)c->nmol * 9, 0);
-    for (int m = 0; m < c->nmol; ++m) {
-      const MolPart& mp = c->mols[m];
-      std::vector<int4> items[3][3];
-      for (int i = mp.sh0; i < mp.sh0 + mp.nsh; ++i)
-        for (int j = mp.sh0; j <= i; ++j)
-          for (int pa = 0; pa < nparts(c->shells[i].type); ++pa)
-            for (int pb = 0; pb < nparts(c->shells[j].type); ++pb)
-              items[part_l(c->shells[i].type, pa)][part_l(c->shells[j].type, pb)].push_back(make_int4(i, j, pa, pb));
-      for (int la = 0; la < 3; ++la)
-        for (int lb = 0; lb < 3; ++lb) {
-          c->onee_count[(size_t)m * 9 + la * 3 + lb] = (int)items[la][lb].size();
-          all.insert(all.end(), items[la][lb].begin(), items[la][lb].end());
-        }
-    }
-    const int mp = 16;
-    std::vector<int> ty(nsh), np(nsh), ao(nsh);
-    std::vector<double> xyz(3 * nsh), z((size_t)mp * nsh), cc((size_t)mp * nsh), cl((size_t)mp * nsh);
-    for (int i = 0; i < nsh; ++i) {
-      const Shell& sh = c->shells[i];
-      ty[i] = sh.type;
-      np[i] = sh.nprim;
-      ao[i] = sh.ao;
-      for (int k = 0; k < 3; ++k) xyz[3 * i + k] = sh.x[k];
-      for (int k = 0; k < mp; ++k) {
-        z[(size_t)mp * i + k] = sh.zeta[k];
-        cc[(size_t)mp * i + k] = sh.djk[k];
-        cl[(size_t)mp * i + k] = sh.djkl[k];
-      }
-    }
-    if (dalloc(&c->d_onee_items, all.size()) || dalloc(&c->d_sh_type, nsh) || dalloc(&c->d_sh_np, nsh) ||
-        dalloc(&c->d_sh_ao, nsh) || dalloc(&c->d_sh_xyz, xyz.size()) || dalloc(&c->d_sh_z, z.size()) ||
-        dalloc(&c->d_sh_c, cc.size()) || dalloc(&c->d_sh_cl, cl.size()))
+    OneeTables t;
+    onee_tables(c, t);
+    const int mp = kOneeMaxPrim;
+    if (dalloc(&c->d_onee_items, t.items.size()) || dalloc(&c->d_sh_type, nsh) || dalloc(&c->d_sh_np, nsh) ||
+        dalloc(&c->d_sh_ao, nsh) || dalloc(&c->d_sh_xyz, 3 * (size_t)nsh) || dalloc(&c->d_sh_z, (size_t)mp * nsh) ||
+        dalloc(&c->d_sh_c, (size_t)mp * nsh) || dalloc(&c->d_sh_cl, (size_t)mp * nsh))
       return HFX_E_NOMEM;
-    HIPOK(copy_sync(c->stream, c->d_onee_items, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
-    HIPOK(copy_sync(c->stream, c->d_sh_type, ty.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
-    HIPOK(copy_sync(c->stream, c->d_sh_np, np.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
-    HIPOK(copy_sync(c->stream, c->d_sh_ao, ao.data(), nsh * sizeof(int), hipMemcpyHostToDevice));
-    HIPOK(copy_sync(c->stream, c->d_sh_xyz, xyz.data(), xyz.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPOK(copy_sync(c->stream, c->d_sh_z, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPOK(copy_sync(c->stream, c->d_sh_c, cc.data(), cc.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPOK(copy_sync(c->stream, c->d_sh_cl, cl.data(), cl.size() * sizeof(double), hipMemcpyHostToDevice));
+    OneeDev d = {c->d_onee_items, c->d_sh_type, c->d_sh_np, c->d_sh_ao, c->d_sh_xyz, c->d_sh_z, c->d_sh_c, c->d_sh_cl,
+                 nullptr, nullptr, nullptr, nullptr, nullptr};
+    if ((r = onee_upload_tables(c, t, d))) return r;
+    c->onee_count = t.count;
   }
   double *d_q = nullptr, *d_xyz = nullptr, *d_S = nullptr, *d_T = nullptr, *d_V = nullptr;
-  const size_t nmat = (size_t)nb * nb * c->nmol;
-  int r = HFX_OK;
+  const size_t nmat = (size_t)c->nb * c->nb * c->nmol;
+  r = HFX_OK;
   if (dalloc(&d_q, std::max(ntot, 1)) || dalloc(&d_xyz, 3 * std::max(ntot, 1)) || dalloc(&d_S, nmat) ||
       dalloc(&d_T, nmat) || dalloc(&d_V, nmat))
     r = HFX_E_NOMEM;
-  if (r == HFX_OK && ntot > 0) {
-    if (copy_sync(c->stream, d_q, charges, ntot * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        copy_sync(c->stream, d_xyz, coords, 3 * ntot * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-      r = HFX_E_HIP;
-  }
-  if (r == HFX_OK) {   // the padding of smaller molecules' blocks stays zero
-    for (double* x : {d_S, d_T, d_V})
-      hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)nmat, 256, 4096)), dim3(256), 0, c->stream, x, nmat, 0.0);
-  }
   if (r == HFX_OK) {
-    OneeArgs a;
-    memset(&a, 0, sizeof(a));
-    a.sh_type = c->d_sh_type; a.sh_np = c->d_sh_np; a.sh_ao = c->d_sh_ao; a.sh_xyz = c->d_sh_xyz;
-    a.sh_z = c->d_sh_z; a.sh_c = c->d_sh_c; a.sh_cl = c->d_sh_cl; a.maxprim = 16;
-    a.boys64 = c->d_boys64;
-    a.nb = nb;
-    int off = 0, atom0 = 0;
-    for (int m = 0; m < c->nmol && r == HFX_OK; ++m) {
-      const size_t base = (size_t)m * nb * nb;
-      a.charges = d_q + atom0; a.coords = d_xyz + 3 * atom0; a.natom = natom[m];
-      a.S = d_S + base; a.T = d_T + base; a.V = d_V + base;
-      atom0 += natom[m];
-      for (int la = 0; la < 3 && r == HFX_OK; ++la)
-        for (int lb = 0; lb < 3 && r == HFX_OK; ++lb) {
-          const int n = c->onee_count[(size_t)m * 9 + la * 3 + lb];
-          if (n == 0) continue;
-          a.items = c->d_onee_items + off;
-          a.nitems = n;
-          off += n;
-          const int grid = grid_for(n, HFX_NT, c->ncu * 8);
-          const size_t shm = (size_t)HFX_BOYS_ROWS * (la + lb + 8) * sizeof(double);
-          hipLaunchKernelGGL(kOneeST[la][lb], dim3(grid), dim3(HFX_NT), 0, c->stream, a);
-          hipLaunchKernelGGL(kOneeV[la][lb], dim3(grid), dim3(HFX_NT), shm, c->stream, a);
-          if (hipGetLastError() != hipSuccess) r = HFX_E_HIP;
-        }
-    }
+    OneeDev d = {c->d_onee_items, c->d_sh_type, c->d_sh_np, c->d_sh_ao, c->d_sh_xyz, c->d_sh_z, c->d_sh_c, c->d_sh_cl,
+                 d_q, d_xyz, d_S, d_T, d_V};
+    r = onee_run(c, d, c->onee_count, natom, ntot, charges, coords, S, T, V);
   }
-  if (r == HFX_OK &&
-      (hipMemcpyAsync(S, d_S, nmat * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-       hipMemcpyAsync(T, d_T, nmat * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-       hipMemcpyAsync(V, d_V, nmat * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-       hipStreamSynchronize(c->stream) != hipSuccess))
-    r = HFX_E_HIP;
   hipStreamSynchronize(c->stream);   // nothing queued may still use the temporaries
   dfree(d_q); dfree(d_xyz); dfree(d_S); dfree(d_T); dfree(d_V);
   return r;
@@ -1429,6 +1526,46 @@ int hfx_ctx_onee(hfx_ctx* c, int natom, const double* charges, const double* coo
                  double* V) {
   if (!c || natom < 0 || c->nmol != 1) return HFX_E_ARG;
   return hfx_batch_onee(c, &natom, charges, coords, S, T, V);
+}
+
+// The same matrices with every device buffer in `scratch` (the tables,
+// nuclei and S, T, V are laid out there on each call): no allocation.
+int hfx_ctx_onee_noalloc(hfx_ctx* c, int natom, const double* charges, const double* coords, double* S, double* T,
+                         double* V) {
+  if (!c || natom < 0 || c->nmol != 1 || !S || !T || !V) return HFX_E_ARG;
+  int ntot = 0, r;
+  if ((r = onee_natoms(c, &natom, charges, coords, &ntot))) return r;
+  if (!c->scratch) return HFX_E_HIP;   // its address could not be taken on this device
+  OneeTables t;
+  onee_tables(c, t);
+  const size_t nsh = (size_t)c->nsh, nmat = (size_t)c->nb * c->nb, mp = kOneeMaxPrim;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {      // 256-byte aligned pieces of the scratch array
+    const size_t at = off;
+    off += (bytes + 255) / 256 * 256;
+    return at;
+  };
+  const size_t o_items = take(t.items.size() * sizeof(int4)), o_ty = take(nsh * sizeof(int)),
+               o_np = take(nsh * sizeof(int)), o_ao = take(nsh * sizeof(int)), o_xyz = take(3 * nsh * sizeof(double)),
+               o_z = take(mp * nsh * sizeof(double)), o_cc = take(mp * nsh * sizeof(double)),
+               o_cl = take(mp * nsh * sizeof(double)), o_q = take(std::max(ntot, 1) * sizeof(double)),
+               o_cxyz = take(3 * (size_t)std::max(ntot, 1) * sizeof(double)), o_S = take(nmat * sizeof(double)),
+               o_T = take(nmat * sizeof(double)), o_V = take(nmat * sizeof(double));
+  if (off > kScratchBytes) {
+    fprintf(stderr, "hfx_ctx_onee_noalloc: %zu bytes of device buffers needed (nb = %d), the scratch array holds %zu\n",
+            off, c->nb, (size_t)kScratchBytes);
+    return HFX_E_UNSUPPORTED;
+  }
+  unsigned char* base = static_cast<unsigned char*>(c->scratch);
+  OneeDev d = {reinterpret_cast<int4*>(base + o_items), reinterpret_cast<int*>(base + o_ty),
+               reinterpret_cast<int*>(base + o_np), reinterpret_cast<int*>(base + o_ao),
+               reinterpret_cast<double*>(base + o_xyz), reinterpret_cast<double*>(base + o_z),
+               reinterpret_cast<double*>(base + o_cc), reinterpret_cast<double*>(base + o_cl),
+               reinterpret_cast<double*>(base + o_q), reinterpret_cast<double*>(base + o_cxyz),
+               reinterpret_cast<double*>(base + o_S), reinterpret_cast<double*>(base + o_T),
+               reinterpret_cast<double*>(base + o_V)};
+  if ((r = onee_upload_tables(c, t, d))) return r;
+  return onee_run(c, d, t.count, &natom, ntot, charges, coords, S, T, V);
 }
 
 int hfx_fock_initial_diag(hfx_ctx* c, const double* P, double* F, int nb, double thr) {

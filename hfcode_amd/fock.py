@@ -155,6 +155,7 @@ def configure(L):
     L.hfx_pair_info.argtypes = [vp, ctypes.c_int, ip, ip]
     L.hfx_onee.argtypes = [ctypes.POINTER(MolDesc), ctypes.c_int, dp, dp, dp, dp, dp]
     L.hfx_ctx_onee.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp, dp]
+    L.hfx_ctx_onee_noalloc.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp, dp]
     L.hfx_molecule_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
     L.hfx_molecule_dims.argtypes = [vp, ip, ip, ip, ip, dp]
     L.hfx_molecule_options.argtypes = [vp, ip, ip, dp, ip]

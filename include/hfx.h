@@ -185,6 +185,18 @@ int hfx_onee(const hfx_mol_desc* desc, int natom, const double* charges, const d
 int hfx_ctx_onee(hfx_ctx* ctx, int natom, const double* charges, const double* coords, double* S, double* T,
                  double* V);
 
+/* hfx_ctx_onee without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Every device buffer
+ * (S, T, V 24 nb^2 bytes, work items 8 to 32 nsh^2 bytes, shell tables
+ * 420 nsh bytes, nuclei) lies in the library's fixed 48 MiB scratch array, one per
+ * GPU for the life of the process; a molecule needing more is refused with
+ * HFX_E_UNSUPPORTED (nb up to about 1300).  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (from different threads or
+ * contexts) race.  Single-molecule contexts only; host arrays in and out. */
+int hfx_ctx_onee_noalloc(hfx_ctx* ctx, int natom, const double* charges, const double* coords, double* S, double* T,
+                         double* V);
+
 /* Molecules in the reference's own text formats (host set-up, no HIP calls).
  * hfx_molecule_read parses an input file (read_infile.f90:12-73) and a basis
  * file (read_basfile.f90:12-91, with the normalisation of :100-179), both

@@ -30,6 +30,8 @@
 #define __forceinline__ inline
 #define __launch_bounds__(x)
 #define __restrict__
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
 
 struct dim3 {
   unsigned x, y, z;
@@ -115,6 +117,10 @@ inline hipError_t hipFree(void* p) {
   return hipSuccess;
 }
 inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) { return hipMalloc(p, n); }
+inline hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
+  *p = const_cast<void*>(symbol);
+  return hipSuccess;
+}
 inline hipError_t hipHostFree(void* p) { return hipFree(p); }
 // stream capture: the recorded operations of the graph being captured (per thread)
 typedef std::vector<std::function<void()>> hfx_emu_ops;

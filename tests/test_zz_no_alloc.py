@@ -38,7 +38,7 @@ def test_noalloc_onee_matches_allocating_entry_point():
 
 
 def test_noalloc_onee_refuses_oversized_molecule():
-    mol = M.chain(120, "alkane")          # 2284 basis functions in 6-31G(d): S, T, V exceed the scratch
+    mol = M.chain(120, "alkane")          # 2164 basis functions in 6-31G(d): S, T, V exceed the scratch
     fb = fock.FockBuilder(M.ShellTable(mol, M.load_basis("6-31g(d)")))
     try:
         rc, S, T, V = _noalloc(fb, mol)
