@@ -25,6 +25,7 @@ take (work-list capacity, integral buffers; estimated from its shell pairs
 before any allocation) fits in the budget next to the running jobs.
 """
 import contextlib
+import math
 import threading
 import time
 from concurrent.futures import ThreadPoolExecutor, as_completed
@@ -274,11 +275,18 @@ def group_by_size(nbs, group):
     return [order[k:k + group] for k in range(0, len(order), group)]
 
 
-def run_batched(mols, basis_name, group=64, workers=2, maxiter=50, split=None, use_graphs=True, progress=None):
+def run_batched(mols, basis_name, group=None, workers=2, maxiter=64, split=None, use_graphs=True, progress=None):
     """Solve every molecule of `mols` in size-sorted groups of `group`
-    molecules, `workers` groups at once; returns (results in input order,
-    wall seconds)."""
+    molecules (default: the tuned group size of batch_launch.json, else 64),
+    `workers` groups at once; returns (results in input order, wall seconds).
+    At most 64 iterations: from the core guess the reference's DIIS schedule
+    takes 50 iterations for some long chains even with the reference's own
+    Fock builds."""
     fock.lib()
+    if group is None:
+        from .gen import autotune_batch
+        tab = autotune_batch.load_table()
+        group = int(tab["group"]) if tab else 64
     basis = _basis(basis_name)
     nbs = [_nb_of(m, basis) for m in mols]
     out = [None] * len(mols)
@@ -313,16 +321,18 @@ def group_memory_estimate(sts):
 class BatchServer:
     """Serving loop with continuous batching.  Requests (arrival time,
     molecule, basis) queue as they arrive; whenever one of `workers` slots is
-    free, the queued requests of one basis (oldest first, at most `group`,
-    within the memory budget next to the running groups) start as one
-    micro-batch: one batch context and one batched SCF, so their builds share
+    free, the oldest queued request and the queued ones of its basis closest
+    to it in size (basis sizes within `size_ratio`, at most `group`, within
+    the memory budget next to the running groups) start as one micro-batch: one batch context and one batched SCF, so their builds share
     every launch.  Micro-batches overlap on the GPU on their own streams; a
     request's result is reported as soon as its own SCF finishes, while the
     rest of its micro-batch goes on.  Latency = completion - arrival."""
 
-    def __init__(self, workers=4, group=32, mem_budget=200 * (1 << 30), maxiter=50, split=None, use_graphs=True):
+    def __init__(self, workers=4, group=32, mem_budget=200 * (1 << 30), maxiter=50, split=None, use_graphs=True,
+                 size_ratio=2.0):
         self.workers = workers
         self.group = group
+        self.size_ratio = size_ratio
         self.mem_budget = mem_budget
         self.maxiter = maxiter
         self.split = split
@@ -375,11 +385,18 @@ class BatchServer:
                                 th.join(0)
                             return threads
                         if queue and state["running"] < self.workers:
-                            basis = queue[0][3]
+                            # the oldest request and the queued ones closest to it in
+                            # size (same basis, basis size within `size_ratio`): a
+                            # micro-batch steps in lockstep, so a small molecule
+                            # should not share it with a much larger one
+                            basis, nb0 = queue[0][3], queue[0][4].nb
+                            cands = [q for q in queue if q[3] == basis and
+                                     max(q[4].nb, nb0) <= self.size_ratio * min(q[4].nb, nb0)]
+                            cands = cands[:1] + sorted(cands[1:], key=lambda q: abs(math.log(q[4].nb / nb0)))
                             batch, need = [], 0
-                            for q in list(queue):
-                                if q[3] != basis or len(batch) >= self.group:
-                                    continue
+                            for q in cands:
+                                if len(batch) >= self.group:
+                                    break
                                 est = group_memory_estimate([b[4] for b in batch] + [q[4]])
                                 if batch and state["mem"] + est > self.mem_budget:
                                     break

@@ -309,6 +309,8 @@ struct hfx_ctx {
   std::vector<CachedGraph> graphs;  // captured builds; dropped whenever a buffer they use is replaced
   bool graphs_off = false;          // a capture failed: later builds run uncaptured
   void* scratch = nullptr;          // address of the scratch array on the context's device
+  int cls_per_cu[HFX_NCLASSES] = {};          // launch width per class (workgroups per CU; 0: default)
+  long long cls_chunk[HFX_NCLASSES] = {};     // quartets per chunk per block class (0: default)
   std::vector<PairRec> pairs;
   std::vector<PrimRec> prims;
   std::vector<ClassPlan> classes;
@@ -532,7 +534,8 @@ int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   const size_t shm = rows ? rows_off + rows_shm : boys_shm;
   a.lds_rows = rows ? 1 : 0;
   a.rows_offset = (int)rows_off;
-  const int per_cu = rows ? (int)std::max<size_t>(2, std::min<size_t>(8, c->max_shm / shm)) : 8;
+  int per_cu = rows ? (int)std::max<size_t>(2, std::min<size_t>(8, c->max_shm / shm)) : 8;
+  if (c->cls_per_cu[cls] > 0) per_cu = c->cls_per_cu[cls];   // tuned launch width
   const int grid = grid_for(cap, HFX_NT, c->ncu * per_cu);
   void* args[] = {&a};
   HIPOK(hipLaunchKernel(fn, dim3(grid), dim3(HFX_NT), args, shm, c->stream));
@@ -601,6 +604,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   }
   long long chunk = std::min(c->vmax_rows, (c->eblk_doubles / ne) / HFX_NT * HFX_NT);
   if (c->chunk_opt > 0) chunk = std::min(chunk, c->chunk_opt);
+  if (c->cls_chunk[cls] > 0) chunk = std::min(chunk, c->cls_chunk[cls]);   // tuned quartets per chunk
   // Several chunks, one live context: the buffers split into two halves and
   // the chunks alternate between them; the combination kernels run on the
   // side streams, the digestion on the context's stream, so chunk j's
@@ -683,7 +687,8 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
         if (load[s] < load[sb]) sb = s;
       load[sb] += cost[ic];
       hipStream_t st = pipelined ? c->side[sb] : (sb == 0 ? c->stream : c->side[sb - 1]);
-      HIPOK(hipLaunchKernel(fn, dim3(grid_for(q1 - q0, HFX_NT, c->ncu * 8)), dim3(HFX_NT), args, shm, st));
+      const int per_cu = c->cls_per_cu[cls] > 0 ? c->cls_per_cu[cls] : 8;   // tuned launch width
+      HIPOK(hipLaunchKernel(fn, dim3(grid_for(q1 - q0, HFX_NT, c->ncu * per_cu)), dim3(HFX_NT), args, shm, st));
     }
     for (int s = 0; s < nside; ++s) {
       HIPOK(hipEventRecord(c->join[s], c->side[s]));
@@ -1299,6 +1304,17 @@ void hfx_ctx_destroy(hfx_ctx* c) {
   if (c->stream) hipStreamDestroy(c->stream);
   if (c->counted) live_contexts.fetch_sub(1);
   delete c;
+}
+
+int hfx_set_class_launch(hfx_ctx* c, int cls, int per_cu, long long chunk) {
+  if (!c || cls < 0 || cls >= HFX_NCLASSES || per_cu < 0 || per_cu > 64 || chunk < 0 || chunk > (1ll << 30))
+    return HFX_E_ARG;
+  if (c->cls_per_cu[cls] != per_cu || c->cls_chunk[cls] != chunk / HFX_NT * HFX_NT) {
+    drop_graphs(c);   // captured launches carry the old shape
+    c->cls_per_cu[cls] = per_cu;
+    c->cls_chunk[cls] = chunk > 0 ? std::max<long long>(HFX_NT, chunk / HFX_NT * HFX_NT) : 0;
+  }
+  return HFX_OK;
 }
 
 int hfx_set_option(hfx_ctx* c, int key, double value) {

@@ -123,6 +123,7 @@ def configure(L):
     L.hfx_release_cached_memory.argtypes = []
     L.hfx_release_cached_memory.restype = None
     L.hfx_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+    L.hfx_set_class_launch.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_longlong]
     for n in ("hfx_fock_nd",):
         getattr(L, n).argtypes = [vp, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int]
     for n in ("hfx_fock_diag", "hfx_fock_initial_diag"):
@@ -435,15 +436,7 @@ class FockBuilder:
                                           ctypes.c_void_p(stream or 0)))
 
     def stats(self):
-        s = Stats()
-        _check(lib().hfx_get_stats(self._h, ctypes.byref(s)))
-        q = np.array([[s.quartets[c][k] for k in range(2)] for c in range(NCLASSES)], dtype=np.int64)
-        pq = np.array([[s.prim_quartets[c][k] for k in range(2)] for c in range(NCLASSES)], dtype=np.int64)
-        bp = np.array([[s.bra_prims[c][k] for k in range(2)] for c in range(NCLASSES)], dtype=np.int64)
-        cf = np.array([[s.class_flops[c][k] for k in range(2)] for c in range(NCLASSES)])
-        cm = np.array([[s.class_ms[c][k] for k in range(2)] for c in range(NCLASSES)])
-        return dict(quartets=q, prim_quartets=pq, bra_prims=bp, flops_fp64=s.flops_fp64,
-                    flops_fp32=s.flops_fp32, ms=s.ms_total, class_flops=cf, class_ms=cm)
+        return _stats(self._h)
 
     def schwarz(self):
         out = []
@@ -475,6 +468,19 @@ class FockBuilder:
 BUILD_ZERO_F, BUILD_GRAPH = 1, 2
 
 
+def _stats(h):
+    """hfx_get_stats of a context as a dict of arrays."""
+    s = Stats()
+    _check(lib().hfx_get_stats(h, ctypes.byref(s)))
+    q = np.array([[s.quartets[c][k] for k in range(2)] for c in range(NCLASSES)], dtype=np.int64)
+    pq = np.array([[s.prim_quartets[c][k] for k in range(2)] for c in range(NCLASSES)], dtype=np.int64)
+    bp = np.array([[s.bra_prims[c][k] for k in range(2)] for c in range(NCLASSES)], dtype=np.int64)
+    cf = np.array([[s.class_flops[c][k] for k in range(2)] for c in range(NCLASSES)])
+    cm = np.array([[s.class_ms[c][k] for k in range(2)] for c in range(NCLASSES)])
+    return dict(quartets=q, prim_quartets=pq, bra_prims=bp, flops_fp64=s.flops_fp64,
+                flops_fp32=s.flops_fp32, ms=s.ms_total, class_flops=cf, class_ms=cm)
+
+
 class BatchBuilder:
     """Several molecules in one context (``hfx_batch_create``): each quartet
     class is screened and evaluated for all of them in one launch of its
@@ -483,7 +489,9 @@ class BatchBuilder:
     (:meth:`stack` / :meth:`unstack`).  The Fock-build methods take and
     return such stacks, with the meaning of :class:`FockBuilder`'s."""
 
-    def __init__(self, shell_tables):
+    def __init__(self, shell_tables, tuned=True):
+        """`tuned`: apply the launch shapes of hfcode_amd/gen/batch_launch.json
+        (autotune_batch.py) when that table exists."""
         L = lib()
         n = len(shell_tables)
         if n < 1:
@@ -503,6 +511,26 @@ class BatchBuilder:
         self.nmol = n
         self.nb = L.hfx_ctx_nb(h)
         self.split = 0.0
+        if tuned:
+            from .gen import autotune_batch
+            tab = autotune_batch.load_table()
+            if tab:
+                for ci, c in enumerate(CLASSES):
+                    e = tab["classes"].get(c)
+                    if e:
+                        self.set_class_launch(ci, e["per_cu"], e["chunk"])
+
+    def set_class_launch(self, cls, per_cu, chunk):
+        """Launch shape of class `cls` (index or name): workgroups per CU and
+        quartets per chunk (0: defaults), hfx_set_class_launch."""
+        ci = CLASSES.index(cls) if isinstance(cls, str) else int(cls)
+        _check(lib().hfx_set_class_launch(self._h, ci, int(per_cu), int(chunk)))
+
+    def set_profile(self, on=True):
+        _check(lib().hfx_set_option(self._h, OPT_PROFILE, 1.0 if on else 0.0))
+
+    def stats(self):
+        return _stats(self._h)
 
     def close(self):
         if getattr(self, "_h", None):

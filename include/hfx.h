@@ -89,6 +89,12 @@ void hfx_ctx_destroy(hfx_ctx* ctx);
  * other jobs run); this hands the cached blocks back to HIP. */
 void hfx_release_cached_memory(void);
 int hfx_set_option(hfx_ctx* ctx, int key, double value);
+/* Launch shape of class cls (0..45) on this context: per_cu workgroups per
+ * compute unit at most for its kernels (0: the library's default), and for
+ * the block classes at most `chunk` quartets per chunk of stored integrals
+ * (0: as many as the integral buffer holds).  Tuning only: results do not
+ * depend on it (hfcode_amd/gen/autotune_batch.py picks them per class). */
+int hfx_set_class_launch(hfx_ctx* ctx, int cls, int per_cu, long long chunk);
 
 /* F += off-diagonal two-electron part of the Fock matrix.
  * Replaces fock_build_non_diag(f,p,thr) (mode HFX_JK) and
