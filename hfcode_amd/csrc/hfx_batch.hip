@@ -117,7 +117,7 @@ enum { kDot3 = 0, kMaxAbs = 1, kMaxAbsDiff = 2, kDotRing = 3 };
 template <int OP>
 __global__ void kb_reduce1(const double* a, const double* b, const double* c, long long n2, long long ring_stride,
                            int nmol, int nz, int nvb, double* part) {
-  __shared__ double sm[kNT];
+  static __shared__ double sm[kNT];
   for (int z = blockIdx.z; z < nz; z += gridDim.z)
     for (int m = blockIdx.y; m < nmol; m += gridDim.y) {
       const double* am = a + (long long)m * n2;
@@ -149,7 +149,7 @@ __global__ void kb_reduce1(const double* a, const double* b, const double* c, lo
 // stage 2: scal[m * kSlots + slot0 + z] = reduce over the partials of (m, z)
 template <int OP>
 __global__ void kb_reduce2(const double* part, int nper, int nmol, int nz, double* scal, int slot0) {
-  __shared__ double sm[kNT];
+  static __shared__ double sm[kNT];
   for (int y = blockIdx.x; y < nmol * nz; y += gridDim.x) {
     const int z = y / nmol, m = y - z * nmol;
     const double* row = part + (long long)y * nper;

@@ -50,7 +50,7 @@ struct Ptrs {
 // per operand, each a partial (sum or max) into part[y * nvb + vb].
 template <int OP>
 __global__ void k_reduce1(const double* a, Ptrs b, const double* c, long long n, int ny, int nvb, double* part) {
-  __shared__ double sm[kRedThreads];
+  static __shared__ double sm[kRedThreads];
   for (int y = blockIdx.y; y < ny; y += gridDim.y) {
     const double* bb = b.p[y];
     for (int vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
@@ -78,7 +78,7 @@ __global__ void k_reduce1(const double* a, Ptrs b, const double* c, long long n,
 // Stage 2: out[slot0 + y] = reduce(part[y * nper .. y * nper + nper)), y < ny.
 template <int OP>
 __global__ void k_reduce2(const double* part, int nper, int ny, double* out, int slot0) {
-  __shared__ double sm[kRedThreads];
+  static __shared__ double sm[kRedThreads];
   for (int y = blockIdx.x; y < ny; y += gridDim.x) {
     const double* row = part + (size_t)y * nper;
     double acc = 0.0;

@@ -5,7 +5,8 @@
 //
 // Every launch runs as one workgroup of one thread (HFX_WAVE = 1) per
 // blockIdx.y (the kernels are grid-stride loops in x, so one thread visits
-// every work item).  Work launched while a stream is being captured is
+// every work item); built with HFX_EMU_LANES, as one workgroup of the
+// launch's blockDim.x lanes in 64-lane waves (below).  Work launched while a stream is being captured is
 // recorded instead and run again by hipGraphLaunch, as a HIP graph would.
 // This library is loaded only by tests; the product (hfcode_amd/fock.py)
 // loads the gfx950 build libhfx.so and nothing else.
@@ -18,15 +19,28 @@
 #include <functional>
 #include <vector>
 
-#ifndef HFX_WAVE
+#ifdef HFX_EMU_LANES
+// Lanes mode (tests/cpuemu/libhfx_lanes.so): every workgroup runs as
+// blockDim.x lanes, 64 to a wave, each lane a fiber on the calling host
+// thread.  A lane runs until it reaches a wave collective (ballot, shuffle,
+// wave barrier) or a block barrier; the collective completes once every
+// lane of the wave (block) that has not exited waits at the same call site,
+// with the GPU's results (exited lanes are inactive: no ballot bit).  Lanes
+// of a wave at different collectives, or a shuffle reading an exited lane,
+// abort with a message.  So code in which lanes see each other (ballot
+// compaction, __syncthreads_count votes, LDS written by one lane and read by
+// another between wave barriers) runs with its 64-lane semantics.
+#define HFX_WAVE 64
+#else
 #define HFX_WAVE 1
-#define HFX_PIN(x) asm volatile("" : "+m"(x))
 #endif
+#define HFX_PIN(x) asm volatile("" : "+m"(x))
 
 #define __global__
 #define __device__
 #define __host__
-#define __shared__
+// LDS: one copy per host thread (the sources declare it static or extern)
+#define __shared__ __thread
 #define __forceinline__ inline
 #define __launch_bounds__(x)
 #define __restrict__
@@ -49,6 +63,12 @@ inline int4 make_int4(int a, int b, int c, int d) { return int4{a, b, c, d}; }
 struct hfx_emu_idx {
   unsigned x, y, z;
 };
+inline int __ffsll(long long m) { return m ? __builtin_ffsll(m) : 0; }
+inline int __popcll(unsigned long long m) { return __builtin_popcountll(m); }
+inline int __clzll(long long m) { return m ? __builtin_clzll((unsigned long long)m) : 64; }
+inline long long __double_as_longlong(double x) { long long r; std::memcpy(&r, &x, 8); return r; }
+inline double __longlong_as_double(long long x) { double r; std::memcpy(&r, &x, 8); return r; }
+#ifndef HFX_EMU_LANES
 static const hfx_emu_idx threadIdx = {0, 0, 0};
 static const hfx_emu_idx blockDim = {1, 1, 1};
 extern thread_local hfx_emu_idx blockIdx;   // x = 0; y runs over the launch's grid.y
@@ -60,9 +80,6 @@ inline int __syncthreads_count(int p) { return p ? 1 : 0; }
 #define __builtin_amdgcn_wave_barrier() do { } while (0)
 inline unsigned long long __ballot(bool p) { return p ? 1ull : 0ull; }
 inline int __lane_id() { return 0; }
-inline int __ffsll(long long m) { return m ? __builtin_ffsll(m) : 0; }
-inline int __popcll(unsigned long long m) { return __builtin_popcountll(m); }
-inline int __clzll(long long m) { return m ? __builtin_clzll((unsigned long long)m) : 64; }
 template <typename T> inline T __shfl(T v, int) { return v; }
 template <typename T> inline T __shfl_xor(T v, int) { return v; }
 template <typename T> inline T __shfl_up(T v, unsigned) { return v; }
@@ -72,13 +89,52 @@ template <typename T> inline T atomicMax(T* p, T v) {
   if (v > old) *p = v;
   return old;
 }
-inline long long __double_as_longlong(double x) { long long r; std::memcpy(&r, &x, 8); return r; }
-inline double __longlong_as_double(long long x) { double r; std::memcpy(&r, &x, 8); return r; }
 template <typename T> inline T atomicAdd(T* p, T v) {
   T old = *p;
   *p = old + v;
   return old;
 }
+#else
+extern thread_local hfx_emu_idx threadIdx;
+extern thread_local hfx_emu_idx blockDim;
+extern thread_local hfx_emu_idx blockIdx;   // x = 0; y runs over the launch's grid.y
+extern thread_local hfx_emu_idx gridDim;    // x = 1
+enum { HFX_LC_BALLOT = 1, HFX_LC_SHFL, HFX_LC_WAVE_BARRIER, HFX_LC_SYNC, HFX_LC_SYNC_COUNT };
+// the calling lane's part in a collective at source line `site`; returns
+// its result once the collective has completed
+unsigned long long hfx_lane_collective(int kind, int site, unsigned long long v, int src);
+// runs f as one workgroup of nt lanes (blockIdx.y = y of gy)
+void hfx_lane_run_block(unsigned nt, unsigned y, unsigned gy, const std::function<void()>& f);
+inline int __lane_id() { return (int)(threadIdx.x & 63u); }
+template <typename T>
+inline T hfx_lane_shfl(T v, int src, int site) {
+  static_assert(sizeof(T) <= 8, "shuffle of more than 8 bytes");
+  unsigned long long b = 0;
+  std::memcpy(&b, &v, sizeof(T));
+  b = hfx_lane_collective(HFX_LC_SHFL, site, b, src & 63);
+  T r;
+  std::memcpy(&r, &b, sizeof(T));
+  return r;
+}
+#define __ballot(p) hfx_lane_collective(HFX_LC_BALLOT, __LINE__, (p) ? 1ull : 0ull, 0)
+#define __shfl(v, s) hfx_lane_shfl((v), (int)(s), __LINE__)
+#define __shfl_xor(v, m) hfx_lane_shfl((v), __lane_id() ^ (int)(m), __LINE__)
+#define __syncthreads() ((void)hfx_lane_collective(HFX_LC_SYNC, __LINE__, 0, 0))
+#define __syncthreads_count(p) ((int)hfx_lane_collective(HFX_LC_SYNC_COUNT, __LINE__, (p) ? 1ull : 0ull, 0))
+#define __builtin_amdgcn_fence(order, scope) do { } while (0)
+#define __builtin_amdgcn_wave_barrier() ((void)hfx_lane_collective(HFX_LC_WAVE_BARRIER, __LINE__, 0, 0))
+// lanes run one at a time on one host thread: plain read-modify-writes
+template <typename T> inline T atomicMax(T* p, T v) {
+  T old = *p;
+  if (v > old) *p = v;
+  return old;
+}
+template <typename T> inline T atomicAdd(T* p, T v) {
+  T old = *p;
+  *p = old + v;
+  return old;
+}
+#endif
 using std::fabs;
 using std::fma;
 using std::fmax;
@@ -86,8 +142,6 @@ using std::sqrt;
 using std::min;
 using std::max;
 
-// dynamic LDS of the class kernels
-extern char hfx_smem[];
 #define HFX_EMU_SMEM_BYTES (1 << 20)
 
 // ---- runtime API ------------------------------------------------------------
@@ -128,9 +182,19 @@ inline hfx_emu_ops*& hfx_emu_capture() {
   static thread_local hfx_emu_ops* cur = nullptr;
   return cur;
 }
-// runs f once per blockIdx.y of the grid, or records that for a graph
-inline void hfx_emu_launch(unsigned gy, std::function<void()> f) {
-  auto run = [gy, f]() {
+// runs f once per blockIdx.y of the grid (lanes mode: a kernel as `threads`
+// lanes per block; threads = 1 is the stand-in libraries' host work), or
+// records that for a graph
+inline void hfx_emu_launch(unsigned gy, std::function<void()> f, unsigned threads = 1) {
+  auto run = [gy, f, threads]() {
+#ifdef HFX_EMU_LANES
+    if (threads > 1) {
+      for (unsigned y = 0; y < gy; ++y) hfx_lane_run_block(threads, y, gy, f);
+      return;
+    }
+#else
+    (void)threads;
+#endif
     const hfx_emu_idx b = blockIdx, g = gridDim;
     gridDim = hfx_emu_idx{1, gy, 1};
     for (unsigned y = 0; y < gy; ++y) {
@@ -227,10 +291,10 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
 }
 // the arguments are evaluated (and copied) at the launch, as HIP does
 template <typename K, typename... A>
-inline void hfx_emu_kernel(dim3 g, K k, A... a) {
-  hfx_emu_launch(g.y, [=]() { k(a...); });
+inline void hfx_emu_kernel(dim3 g, dim3 b, K k, A... a) {
+  hfx_emu_launch(g.y, [=]() { k(a...); }, b.x);
 }
-#define hipLaunchKernelGGL(k, g, b, shm, st, ...) hfx_emu_kernel(dim3(g), k, __VA_ARGS__)
+#define hipLaunchKernelGGL(k, g, b, shm, st, ...) hfx_emu_kernel(dim3(g), dim3(b), k, __VA_ARGS__)
 inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t) {
   *n = 1;
   return hipSuccess;
@@ -245,12 +309,12 @@ struct hfx_emu_argbuf {
   alignas(16) char b[512];
 };
 size_t hfx_emu_arg_size(const void* fn);
-inline hipError_t hipLaunchKernel(const void* fn, dim3 g, dim3, void** args, size_t shm, hipStream_t) {
+inline hipError_t hipLaunchKernel(const void* fn, dim3 g, dim3 b, void** args, size_t shm, hipStream_t) {
   if (shm > HFX_EMU_SMEM_BYTES) return 3;
   hfx_emu_argbuf buf;
   std::memset(buf.b, 0, sizeof(buf.b));
   std::memcpy(buf.b, args[0], hfx_emu_arg_size(fn));
   typedef void (*K)(hfx_emu_argbuf);
-  hfx_emu_launch(g.y, [fn, buf]() { ((K)fn)(buf); });
+  hfx_emu_launch(g.y, [fn, buf]() { ((K)fn)(buf); }, b.x);
   return hipSuccess;
 }
