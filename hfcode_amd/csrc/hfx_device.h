@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #define HFX_NT 256                 // threads per workgroup of the class kernels
+#define DIG1_NT 512                // threads per workgroup of the one-pass digestion (two waves per SIMD)
 #ifndef HFX_WAVE
 #define HFX_WAVE 64                // wavefront width of gfx950
 #endif

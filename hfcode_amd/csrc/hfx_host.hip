@@ -39,6 +39,9 @@ const double kNegab = 1.0e-12;     // extrathresh = intthresh*0.01 (precalc_kabs
 // kMaxShellPrims^2 + 1 < 128.
 const int kMaxShellPrims = 10;
 static_assert(kMaxShellPrims * kMaxShellPrims + 1 < 128, "k_screen's primitive-count set holds keys below 128");
+// Screening statistics per class and list: surviving quartets, primitive
+// quartets, bra primitive iterations, bra pairs with a survivor (runs).
+const int kStatN = 4;
 
 int letter(int shtyp) {  // canonical pair-name letter order s < p < d < L
   switch (shtyp) {
@@ -326,6 +329,8 @@ struct hfx_ctx {
   long long chunk_opt = 0;      // HFX_OPT_CHUNK (0: as large as the buffer allows)
   int class_ran[2][HFX_NCLASSES] = {};
   int dig_occ[3][HFX_NCLASSES] = {};   // resident digestion workgroups per CU [fp64, fp64 J-only, fp32]
+  int digest1 = 1;                     // HFX_OPT_DIGEST: one-pass digestion where its LDS rows fit
+  int dig1_occ[2][HFX_NCLASSES] = {};  // resident one-pass digestion workgroups per CU [fp64, fp32]
   hipEvent_t cev[2][HFX_NCLASSES][2] = {};
   int last_kind = 0;  // 0 none, 1 non-diag, 2 diag
   // device
@@ -370,7 +375,7 @@ struct hfx_ctx {
   int *d_sh_type = nullptr, *d_sh_np = nullptr, *d_sh_ao = nullptr;
   double *d_sh_xyz = nullptr, *d_sh_z = nullptr, *d_sh_c = nullptr, *d_sh_cl = nullptr;
   int* d_counts = nullptr;                     // [2][HFX_NCLASSES]
-  unsigned long long* d_stats = nullptr;       // [2][HFX_NCLASSES][3]
+  unsigned long long* d_stats = nullptr;       // [2][HFX_NCLASSES][kStatN]
 };
 
 #define HIPOK(x)                                                              \
@@ -548,6 +553,7 @@ const long long kEblkDoubles = (1ll << 20) * 625;       // integral buffer of th
 const long long kEblkDoublesAlone = (1ll << 21) * 625;  // ... or this when it grows in the one live context (10.5 GB)
 const long long kMaxChunk = 1 << 22;                      // quartets per chunk, at most
 const int kMaxCombo = 16;                            // combination kernels per class, at most
+const long long kDig1MinRun = 1024;   // one-pass digestion: surviving quartets per bra run, at least (mean)
 
 const long long kConcurrentChunk = 1 << 17;   // quartets of a chunk from which its combinations run side by side
 std::atomic<int> live_contexts{0};             // contexts created and not yet destroyed, process-wide
@@ -577,7 +583,8 @@ int ensure_side_streams(hfx_ctx* c) {
 // list and, where the workgroup's LDS holds the rows of X that a bra pair's
 // quartets update (rows of shells A and B: J_ab, K_ac, K_ad, K_bc, K_bd),
 // accumulates them there and flushes them with row-coalesced atomics.
-int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long long n, double* dump) {
+int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long long n, double* dump,
+                 long long runs = 0) {
   const ClassKernels& k = kClassKernels[cls];
   if (n <= 0) return HFX_OK;
   const long long ne = (long long)k.dims[0] * k.dims[1] * k.dims[2] * k.dims[3];
@@ -633,6 +640,24 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   const size_t tb_shm = (size_t)(HFX_NT / HFX_WAVE) * std::min(HFX_WAVE, 16) * k.dims[2] * k.dims[3] * sizeof(double);
   const bool rows = !a.diag && rows_shm + tb_shm <= c->max_shm;
   const size_t dig_shm = (rows ? rows_shm : 0) + tb_shm;
+  // one-pass digestion (one molecule, J and K, off-diagonal): X rows, then the
+  // same rows of P in the blocks' precision, the X_cd staging and one int per wave
+  const size_t one_prow_off = rows_shm;
+  const size_t one_tb_off = (one_prow_off + (size_t)(k.dims[0] + k.dims[1]) * c->nb *
+                             (prec == 1 ? sizeof(float) : sizeof(double)) + 15) / 16 * 16;
+  const size_t one_run_off = one_tb_off + (size_t)(DIG1_NT / HFX_WAVE) * std::min(HFX_WAVE, 16) * k.dims[2] *
+                                              k.dims[3] * sizeof(double);
+  const size_t one_shm = one_run_off + (size_t)(DIG1_NT / HFX_WAVE) * sizeof(int);
+  // where it measured faster (the class's table entry for the list's
+  // precision) and the list's bra runs are long (a step never crosses a bra
+  // pair, so short runs leave most of the workgroup idle); runs = 0: unknown
+  // (launches sized on the device), the table alone decides; HFX_OPT_DIGEST
+  // 2 takes it wherever it fits
+  const bool one_ok = c->digest1 == 2 ||
+                      (c->digest1 == 1 && (k.dig1_use >> (prec == 1 ? 1 : 0) & 1) &&
+                       (runs <= 0 || n >= kDig1MinRun * runs));
+  const bool one_pass = one_ok && !a.diag && !jonly && c->nmol == 1 && dump == nullptr &&
+                        a.schw_out == nullptr && k.dig1 != nullptr && one_shm <= c->max_shm;
   const int ncombo = prec == 1 ? k.ncombo32 : k.ncombo;
   const bool prof = c->profile && !c->device_chunks;
   if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
@@ -725,6 +750,26 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     // FP32 quartets: blocks stored as floats and contracted with the FP32 copy
     // of P; every quartet's sums are added to X in FP64
     const bool multi = c->nmol > 1;   // molecule-aware form only for batch contexts
+    if (one_pass) {
+      // one pass over each block, rows A and B of X and of P in LDS; a
+      // workgroup per resident slot, each a contiguous range of the chunk
+      g.prow_offset = (int)one_prow_off;
+      g.tb_offset = (int)one_tb_off;
+      g.run_offset = (int)one_run_off;
+      const void* fn1 = prec == 1 ? k.dig1f : k.dig1;
+      int& occ1 = c->dig1_occ[prec == 1 ? 1 : 0][cls];
+      if (occ1 == 0) {
+        int nblk = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, fn1, DIG1_NT, one_shm) != hipSuccess || nblk < 1) {
+          (void)hipGetLastError();
+          nblk = 1;
+        }
+        occ1 = std::min(nblk, 4);
+      }
+      HIPOK(hipLaunchKernel(fn1, dim3(c->ncu * occ1), dim3(DIG1_NT), gargs, one_shm, c->stream));
+      if (pipelined) HIPOK(hipEventRecord(c->done[h], c->stream));
+      continue;
+    }
     const void* dfn = prec == 1 ? (multi ? k.digfm : k.digf) : (jonly ? (multi ? k.digjm : k.digj) : (multi ? k.digm : k.dig));
     // with LDS rows: one workgroup per resident slot (registers and LDS, as
     // the runtime reports them; at most four per CU), each a contiguous range
@@ -755,7 +800,7 @@ int zero_build_state(hfx_ctx* c) {
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((long long)npb, 256, 2048)), dim3(256), 0, c->stream, c->d_pairbuf, npb,
                      0.0);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, c->stream, c->d_counts, 2 * HFX_NCLASSES, 0);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, c->stream, c->d_stats, 2 * HFX_NCLASSES * 3);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, c->stream, c->d_stats, 2 * HFX_NCLASSES * kStatN);
   hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(c->nmol, 256, 64)), dim3(256), 0, c->stream, c->d_pmax, c->nmol);
   HIPOK(hipGetLastError());
   return HFX_OK;
@@ -827,8 +872,8 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
     s.list32 = mixed ? cp.list32 : nullptr;
     s.count64 = c->d_counts + cp.cls;
     s.count32 = c->d_counts + HFX_NCLASSES + cp.cls;
-    s.stats64 = c->d_stats + 3 * cp.cls;
-    s.stats32 = c->d_stats + 3 * (HFX_NCLASSES + cp.cls);
+    s.stats64 = c->d_stats + kStatN * cp.cls;
+    s.stats32 = c->d_stats + kStatN * (HFX_NCLASSES + cp.cls);
     s.thr = thr;
     s.split = mixed ? split : 0.0;
     s.nbra = c->act_len[cp.t1];
@@ -847,11 +892,13 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
   }
   // surviving-quartet counts of the block classes decide their chunking
   int hcounts[2 * HFX_NCLASSES] = {0};
+  unsigned long long hstats[2 * HFX_NCLASSES * kStatN] = {0};   // bra runs pick the digestion form
   bool any_split = false;
   for (auto& cp : c->classes)
     if (cp.cap > 0 && kClassKernels[cp.cls].split) any_split = true;
   if (any_split && !c->device_chunks) {
     HIPOK(hipMemcpyAsync(hcounts, c->d_counts, sizeof(hcounts), hipMemcpyDeviceToHost, c->stream));
+    HIPOK(hipMemcpyAsync(hstats, c->d_stats, sizeof(hstats), hipMemcpyDeviceToHost, c->stream));
     HIPOK(hipStreamSynchronize(c->stream));
   }
   for (auto& cp : c->classes) {
@@ -868,7 +915,8 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
       }
       if (kClassKernels[cp.cls].split)
         r = launch_split(c, cp.cls, prec, jonly, a,
-                         c->device_chunks ? cp.cap : hcounts[prec * HFX_NCLASSES + cp.cls], a.blocks);
+                         c->device_chunks ? cp.cap : hcounts[prec * HFX_NCLASSES + cp.cls], a.blocks,
+                         c->device_chunks ? 0 : (long long)hstats[kStatN * (prec * HFX_NCLASSES + cp.cls) + 3]);
       else
         r = launch_class(c, cp.cls, prec, jonly, a, cp.cap);
       if (r) return r;
@@ -905,7 +953,7 @@ int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initia
     s.thr_mol = thr_mol;
     s.list = c->d_diag_list[t];
     s.count = c->d_counts + cls;
-    s.stats = c->d_stats + 3 * cls;
+    s.stats = c->d_stats + kStatN * cls;
     s.thr = thr;
     s.first = c->offset[t];
     s.npair = n;
@@ -1241,7 +1289,7 @@ int hfx_batch_create(const hfx_mol_desc* descs, int nmol, hfx_ctx** out) {
       (r = dalloc(&c->d_boys32, b32.size())) || (r = dalloc(&c->d_P, nmat)) || (r = dalloc(&c->d_F, nmat)) ||
       (r = dalloc(&c->d_X, nmat)) || (r = dalloc(&c->d_maxdm, (size_t)c->nsp * c->nsp * nmol)) ||
       (r = dalloc(&c->d_pmax, nmol)) || (r = dalloc(&c->d_schw_out, (size_t)tot)) ||
-      (r = dalloc(&c->d_counts, 2 * HFX_NCLASSES)) || (r = dalloc(&c->d_stats, 2 * HFX_NCLASSES * 3)))
+      (r = dalloc(&c->d_counts, 2 * HFX_NCLASSES)) || (r = dalloc(&c->d_stats, 2 * HFX_NCLASSES * kStatN)))
     return fail(r);
   for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
     if (c->npair[t] == 0) continue;
@@ -1332,6 +1380,12 @@ int hfx_set_option(hfx_ctx* c, int key, double value) {
     if (!(value >= 0.0) || value > 1e12) return HFX_E_ARG;
     c->chunk_opt = (long long)value / HFX_NT * HFX_NT;
     if (value > 0.0 && c->chunk_opt == 0) c->chunk_opt = HFX_NT;
+    return HFX_OK;
+  }
+  if (key == HFX_OPT_DIGEST) {
+    if (value != 0.0 && value != 1.0 && value != 2.0) return HFX_E_ARG;
+    if (c->digest1 != (int)value) drop_graphs(c);   // captured launches carry the old kernels
+    c->digest1 = (int)value;
     return HFX_OK;
   }
   if (key == HFX_OPT_PROFILE) {
@@ -1717,15 +1771,16 @@ int hfx_get_stats(hfx_ctx* c, hfx_stats* out) {
   memset(out, 0, sizeof(*out));
   if (!c->last_kind) return HFX_OK;
   HIPOK(hipStreamSynchronize(c->stream));
-  std::vector<unsigned long long> st(2 * HFX_NCLASSES * 3);
+  std::vector<unsigned long long> st(2 * HFX_NCLASSES * kStatN);
   HIPOK(copy_sync(c->stream, st.data(), c->d_stats, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   for (int cl = 0; cl < HFX_NCLASSES; ++cl) {
     const ClassKernels& k = kClassKernels[cl];
     for (int p = 0; p < 2; ++p) {
-      const unsigned long long* s = &st[3 * (p * HFX_NCLASSES + cl)];
+      const unsigned long long* s = &st[kStatN * (p * HFX_NCLASSES + cl)];
       out->quartets[cl][p] = (long long)s[0];
       out->prim_quartets[cl][p] = (long long)s[1];
       out->bra_prims[cl][p] = (long long)s[2];
+      out->bra_runs[cl][p] = (long long)s[3];
       const double f = (double)s[1] * (p ? k.flops_prim_fp32 : k.flops_prim) + (double)s[2] * k.flops_braprim;
       out->class_flops[cl][p] = f + (double)s[0] * k.flops_quartet;
       if (c->profile && c->class_ran[p][cl]) {

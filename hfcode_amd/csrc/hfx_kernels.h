@@ -22,7 +22,7 @@ struct ScreenArgs {
   int2* list32;
   int* count64;
   int* count32;
-  unsigned long long* stats64; // [quartets, primitive quartets, bra primitives]
+  unsigned long long* stats64; // [quartets, primitive quartets, bra primitives, bra pairs with a survivor]
   unsigned long long* stats32;
   const unsigned long long* pmax;  // per molecule: max over its maxdm, bits of a non-negative double
   unsigned long long* masks;   // survivor bits: [bra][chunk of HFX_WAVE kets][list 64, list 32]
@@ -69,6 +69,8 @@ struct DigestArgs {
   int dims[4];                 // components of the oriented shells A, B, C, D
   int swap_bra, swap_ket;      // oriented A is the pair's second shell (bra / ket)
   const double* thr_mol;       // block test threshold per molecule (device), or null: thr
+  int prow_offset;             // one-pass digestion: byte offset in LDS of the rows A and B of P
+  int run_offset;              // one-pass digestion: byte offset in LDS of one int per wave
 };
 
 // One-electron kernels (generated, gen/onee.hip): one thread per work item

@@ -199,12 +199,14 @@ __global__ void k_screen(const ScreenArgs s) {
         atomicAdd(&s.stats64[0], (unsigned long long)n64);
         atomicAdd(&s.stats64[1], pq64);
         atomicAdd(&s.stats64[2], bp64);
+        atomicAdd(&s.stats64[3], 1ull);   // one more bra run in the list
       }
       if (n32) {
         base32 = atomicAdd(s.count32, n32);
         atomicAdd(&s.stats32[0], (unsigned long long)n32);
         atomicAdd(&s.stats32[1], pq32);
         atomicAdd(&s.stats32[2], bp32);
+        atomicAdd(&s.stats32[3], 1ull);
       }
     }
     base64 = __shfl(base64, 0);

@@ -29,7 +29,7 @@ CLASSES = ("ssss sssp sssd sssl sspp sspd sspl ssdd ssdl ssll spsp spsd spsl spp
 NTYPES = 10
 NCLASSES = 46
 HFX_JK, HFX_J_ONLY = 0, 1
-OPT_MIXED_SPLIT, OPT_DEBUG_BLOCKS, OPT_PROFILE, OPT_CHUNK = 1, 2, 3, 4
+OPT_MIXED_SPLIT, OPT_DEBUG_BLOCKS, OPT_PROFILE, OPT_CHUNK, OPT_DIGEST = 1, 2, 3, 4, 5
 ERRORS = {-1: "bad argument", -2: "unsupported basis", -3: "HIP runtime error",
           -4: "device out of memory", -5: "Schwarz tables missing (run initial_diag first)"}
 
@@ -64,6 +64,7 @@ class Stats(ctypes.Structure):
         ("ms_total", ctypes.c_double),
         ("class_flops", (ctypes.c_double * 2) * NCLASSES),
         ("class_ms", (ctypes.c_float * 2) * NCLASSES),
+        ("bra_runs", (ctypes.c_longlong * 2) * NCLASSES),
     ]
 
 
@@ -358,6 +359,12 @@ class FockBuilder:
     def set_debug_blocks(self, on=True):
         _check(lib().hfx_set_option(self._h, OPT_DEBUG_BLOCKS, 1.0 if on else 0.0))
 
+    def set_one_pass_digest(self, mode=1):
+        """Block classes' digestion: 1 (or True) one pass over each integral
+        block, rows of X and P in LDS, where it measured faster and bra runs
+        are long (default); 2 wherever it fits; 0 (or False) multi-pass."""
+        _check(lib().hfx_set_option(self._h, OPT_DIGEST, float(int(mode))))
+
     def _prep(self, f, p):
         nb = self.nb
         p = np.ascontiguousarray(p, dtype=np.float64)
@@ -477,8 +484,9 @@ def _stats(h):
     bp = np.array([[s.bra_prims[c][k] for k in range(2)] for c in range(NCLASSES)], dtype=np.int64)
     cf = np.array([[s.class_flops[c][k] for k in range(2)] for c in range(NCLASSES)])
     cm = np.array([[s.class_ms[c][k] for k in range(2)] for c in range(NCLASSES)])
+    br = np.array([[s.bra_runs[c][k] for k in range(2)] for c in range(NCLASSES)], dtype=np.int64)
     return dict(quartets=q, prim_quartets=pq, bra_prims=bp, flops_fp64=s.flops_fp64,
-                flops_fp32=s.flops_fp32, ms=s.ms_total, class_flops=cf, class_ms=cm)
+                flops_fp32=s.flops_fp32, ms=s.ms_total, class_flops=cf, class_ms=cm, bra_runs=br)
 
 
 class BatchBuilder:

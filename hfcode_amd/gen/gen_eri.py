@@ -1108,6 +1108,221 @@ class Gen:
             W("template __global__ void dig_%s<true, double, %s>(const DigestArgs);" % (cls, multi))
             W("template __global__ void dig_%s<false, float, %s>(const DigestArgs);" % (cls, multi))
 
+    def kernel_digest1(self, W):
+        """One-pass digestion of a block class (single molecule, off-diagonal
+        J and K builds): every element of a block is loaded once and used in
+        all six contractions.  A workgroup of DIG1_NT threads walks a
+        contiguous range of the bra-contiguous list in steps that never cross
+        a bra pair (a step is the run of the current bra's quartets, at most
+        DIG1_NT), so every quartet of a step has the resident bra; the rows of
+        X (accumulated) and of P (read) of shells A and B sit in LDS.  Only
+        P_cd is gathered from global memory.  Per quartet, with the outer
+        shell O of the bra (the larger of A, B) looped at run time and the
+        inner one I unrolled:
+          J:  X_ab += 2w sum_cd e P_cd per (a, b) (summed over the wave into
+              LDS), X_cd += 2w sum_ab e P_ab (staged to the pair buffer);
+          K:  per o: X_oc -= w/2 sum_{i,d} e P_id, X_od -= w/2 sum_{i,c} e P_ic;
+              held over o: X_ic -= w/2 sum_{o,d} e P_od, X_id -= w/2 sum_{o,c} e P_oc."""
+        cls = self.cls
+        nA, nB, nC, nD = self.dims()
+        ne = nA * nB * nC * nD
+        ncd = nC * nD
+        oa = "b" if self.swap_bra else "a"
+        ob = "a" if self.swap_bra else "b"
+        oc = "b" if self.swap_ket else "a"
+        od = "a" if self.swap_ket else "b"
+        a_outer = nA >= nB
+        n_o, n_i = (nA, nB) if a_outer else (nB, nA)
+        # element stride of the outer index, and the LDS row of outer index o / inner index i
+        o_stride = (nB * ncd if a_outer else ncd) * 64
+
+        def lin_i(i, c, d):       # element offset of (o = 0, i, c, d)
+            ia, ib = (0, i) if a_outer else (i, 0)
+            return (((ia * nB + ib) * nC + c) * nD + d) * 64
+
+        orow = "o" if a_outer else "%d + o" % nA
+        irow = (lambda i: "%d" % (nA + i)) if a_outer else (lambda i: "%d" % i)
+        arow = "o" if a_outer else None   # row of a in srow / prow
+
+        def pin(names):
+            W("        " + " ".join("HFX_PIN(%s);" % n for n in names) + " HFX_LOAD_FENCE();")
+
+        W("template <typename T>")
+        W("__global__ void __launch_bounds__(DIG1_NT) dig1_%s(const DigestArgs d) {" % cls)
+        W("  extern __shared__ char hfx_smem[];")
+        W("  constexpr int NR = %d;   // rows of A then rows of B" % (nA + nB))
+        W("  double* srow = reinterpret_cast<double*>(hfx_smem);                 // [NR][nb] of X")
+        W("  T* prow = reinterpret_cast<T*>(hfx_smem + d.prow_offset);           // [NR][nb] of P (P32 for FP32 blocks)")
+        W("  int* wrun = reinterpret_cast<int*>(hfx_smem + d.run_offset);       // per wave: first lane past the bra's run")
+        W("  const int qend = min(d.q1, *d.count);")
+        W("  if (qend <= d.q0) return;")
+        W("  const int nbf = d.nb;")
+        W("  const int lane = threadIdx.x & (HFX_WAVE - 1);")
+        W("  const int nwave = blockDim.x / HFX_WAVE;")
+        W("  double* X = d.X;")
+        W("  const T* PA;")
+        W("  if constexpr (sizeof(T) == sizeof(float)) PA = reinterpret_cast<const T*>(d.P32);")
+        W("  else PA = reinterpret_cast<const T*>(d.P);")
+        W("  const double thq = d.thr_mol ? d.thr_mol[0] : d.thr;")
+        W("  // this workgroup's contiguous range of the chunk")
+        W("  const long long ntot = qend - d.q0;")
+        W("  const int Q0 = d.q0 + (int)(ntot * blockIdx.x / gridDim.x);")
+        W("  const int Q1 = d.q0 + (int)(ntot * (blockIdx.x + 1) / gridDim.x);")
+        W("  for (int e = threadIdx.x; e < NR * nbf; e += blockDim.x) srow[e] = 0.0;")
+        W("  __syncthreads();")
+        W("  int curb = -1, rA = 0, rB = 0;")
+        W("  for (int q = Q0; q < Q1;) {")
+        W("    // the step: the run of list[q].x's quartets from q, at most blockDim.x")
+        W("    const int bra = d.list[q].x;")
+        W("    const int t = q + (int)threadIdx.x;")
+        W("    const bool differ = t >= Q1 || d.list[min(t, Q1 - 1)].x != bra;")
+        W("    const unsigned long long dm = __ballot(differ);")
+        W("    if (lane == 0) wrun[threadIdx.x / HFX_WAVE] = dm ? (int)(threadIdx.x - lane) + __ffsll((long long)dm) - 1"
+          " : (int)blockDim.x;")
+        W("    __syncthreads();")
+        W("    int n = (int)blockDim.x;")
+        W("    for (int w = 0; w < nwave; ++w) n = min(n, wrun[w]);")
+        W("    if (bra != curb) {")
+        W("      if (curb >= 0) {   // X rows of the previous bra to X, row-coalesced")
+        W("        for (int e = threadIdx.x; e < NR * nbf; e += blockDim.x) {")
+        W("          const double v = srow[e];")
+        W("          if (v != 0.0) {")
+        W("            const int r = e / nbf, col = e - r * nbf;")
+        W("            atomicAdd(&X[(size_t)(r < %d ? rA + r : rB + (r - %d)) * nbf + col], v);" % (nA, nA))
+        W("            srow[e] = 0.0;")
+        W("          }")
+        W("        }")
+        W("      }")
+        W("      curb = bra;")
+        W("      rA = d.pairs[bra].ao%s; rB = d.pairs[bra].ao%s;" % (oa, ob))
+        W("      for (int e = threadIdx.x; e < NR * nbf; e += blockDim.x) {")
+        W("        const int r = e / nbf, col = e - r * nbf;")
+        W("        prow[e] = PA[(size_t)(r < %d ? rA + r : rB + (r - %d)) * nbf + col];" % (nA, nA))
+        W("      }")
+        W("    }")
+        W("    __syncthreads();   // rows loaded / flushed, wrun read by every thread")
+        W("    const int it = q + (int)(threadIdx.x - lane);   // first quartet of the wave")
+        W("    q += n;")
+        W("    if (it >= q) continue;   // the whole wave is past the step")
+        W("    const bool valid = t < q;")
+        W("    const int iq = valid ? t : it;")
+        W("    const int slot = iq - d.q0;")
+        W("    const int2 bk = d.list[iq];")
+        W("    double vmax = d.vmax[slot];")
+        W("    for (int c = 1; c < d.ncombo; ++c) vmax = fmax(vmax, d.vmax[(size_t)c * d.vstride + slot]);")
+        W("    const bool act = valid && vmax >= thq;")
+        W("    if (__ballot(act) == 0) continue;")
+        W("    const T* blk = reinterpret_cast<const T*>(d.eblk) + (size_t)(slot >> 6) * %d + (slot & 63);" % (ne * 64))
+        W("    const PairRec& pb = d.pairs[bk.x];")
+        W("    const PairRec& pk = d.pairs[bk.y];")
+        W("    const int shC = pk.sh%s, shD = pk.sh%s;" % (oc, od))
+        W("    const int aoC = pk.ao%s, aoD = pk.ao%s;" % (oc, od))
+        W("    const double w = act ? ((pb.sha == pb.shb) ? 0.5 : 1.0) * ((shC == shD) ? 0.5 : 1.0) : 0.0;")
+        W("    const double wj = 2.0 * w, wk = -0.5 * w;")
+        for c in range(nC):
+            for dd in range(nD):
+                W("    const T Pcd_%d_%d = PA[(size_t)(aoC + %d) * nbf + aoD + %d];" % (c, dd, c, dd))
+        W("    " + " ".join("T Jcd_%d = 0;" % k for k in range(ncd)))
+        for i in range(n_i):
+            W("    " + " ".join("T Kic_%d_%d = 0;" % (i, c) for c in range(nC)) + " "
+              + " ".join("T Kid_%d_%d = 0;" % (i, dd) for dd in range(nD)))
+        W("    #pragma unroll 1")
+        W("    for (int o = 0; o < %d; ++o) {" % n_o)
+        W("      const T* bp = blk + (size_t)o * %d; HFX_PIN(bp);" % o_stride)
+        W("      const T* po = prow + (size_t)(%s) * nbf;   // P row of the outer index" % orow)
+        for dd in range(nD):
+            W("      const T Pod_%d = po[aoD + %d];" % (dd, dd))
+        for c in range(nC):
+            W("      const T Poc_%d = po[aoC + %d];" % (c, c))
+        W("      " + " ".join("T Koc_%d = 0;" % c for c in range(nC)) + " " + " ".join("T Kod_%d = 0;" % dd for dd in range(nD)))
+        for i in range(n_i):
+            W("      {")
+            W("        const T* pi = prow + (size_t)%s * nbf;   // P row of inner index %d" % (irow(i), i))
+            for dd in range(nD):
+                W("        const T Pid_%d = pi[aoD + %d];" % (dd, dd))
+            for c in range(nC):
+                W("        const T Pic_%d = pi[aoC + %d];" % (c, c))
+            # P_ab of (a, b) = (o, i) or (i, o): row a of A, column b of B
+            if a_outer:
+                W("        const T pab = prow[(size_t)o * nbf + rB + %d];" % i)
+            else:
+                W("        const T pab = prow[(size_t)%d * nbf + rB + o];" % i)
+            W("        T jab = 0;")
+            for c in range(nC):
+                for dd in range(nD):
+                    k = c * nD + dd
+                    W("        { const T v = bp[%d]; jab = fma(v, Pcd_%d_%d, jab); Jcd_%d = fma(v, pab, Jcd_%d);"
+                      " Koc_%d = fma(v, Pid_%d, Koc_%d); Kod_%d = fma(v, Pic_%d, Kod_%d);"
+                      " Kic_%d_%d = fma(v, Pod_%d, Kic_%d_%d); Kid_%d_%d = fma(v, Poc_%d, Kid_%d_%d); }" % (
+                          lin_i(i, c, dd), c, dd, k, k, c, dd, c, dd, c, dd,
+                          i, c, dd, i, c, i, dd, c, i, dd))
+            # X_ab of this (a, b): summed over the wave (every lane has the resident bra)
+            W("        double r = wj * (double)jab;")
+            W("        for (int s = HFX_WAVE / 2; s > 0; s >>= 1) r += __shfl_xor(r, s);")
+            if a_outer:
+                W("        if (lane == 0 && r != 0.0) atomicAdd(&srow[(size_t)o * nbf + rB + %d], r);" % i)
+            else:
+                W("        if (lane == 0 && r != 0.0) atomicAdd(&srow[(size_t)%d * nbf + rB + o], r);" % i)
+            pin(["Jcd_%d" % k for k in range(ncd)] + ["Koc_%d" % c for c in range(nC)]
+                + ["Kod_%d" % dd for dd in range(nD)] + ["Kic_%d_%d" % (i, c) for c in range(nC)]
+                + ["Kid_%d_%d" % (i, dd) for dd in range(nD)])
+            W("      }")
+        W("      if (act) {")
+        W("        double* so = srow + (size_t)(%s) * nbf;" % orow)
+        for c in range(nC):
+            W("        atomicAdd(&so[aoC + %d], wk * (double)Koc_%d);" % (c, c))
+        for dd in range(nD):
+            W("        atomicAdd(&so[aoD + %d], wk * (double)Kod_%d);" % (dd, dd))
+        W("      }")
+        W("    }")
+        W("    if (act) {")
+        for i in range(n_i):
+            W("      {")
+            W("        double* si = srow + (size_t)%s * nbf;" % irow(i))
+            for c in range(nC):
+                W("        atomicAdd(&si[aoC + %d], wk * (double)Kic_%d_%d);" % (c, i, c))
+            for dd in range(nD):
+                W("        atomicAdd(&si[aoD + %d], wk * (double)Kid_%d_%d);" % (dd, i, dd))
+            W("      }")
+        W("    }")
+        # X_cd through the wave's LDS staging into the ket pair's slot of the pair buffer
+        W("    {")
+        W("      constexpr int TG = HFX_WAVE < 16 ? HFX_WAVE : 16;")
+        W("      double* tb = reinterpret_cast<double*>(hfx_smem + d.tb_offset) + (threadIdx.x / HFX_WAVE) * (TG * %d);" % ncd)
+        W("      const double sc = act ? wj : 0.0;")
+        W("      for (int r = 0; r < HFX_WAVE / TG; ++r) {")
+        W("        if (lane / TG == r) {")
+        W("          double* tt = tb + (lane - r * TG) * %d;" % ncd)
+        W("          " + " ".join("tt[%d] = sc * (double)Jcd_%d;" % (k, k) for k in range(ncd)))
+        W("        }")
+        W("        wave_sync();")
+        W("        for (int f0 = 0; f0 < TG * %d; f0 += HFX_WAVE) {" % ncd)
+        W("          const int f = f0 + lane;")
+        W("          const bool ok = f < TG * %d;" % ncd)
+        W("          const int ql = ok ? f / %d : 0, e = ok ? f - ql * %d : 0;" % (ncd, ncd))
+        W("          const int pq = __shfl(bk.y, r * TG + ql);")
+        W("          const int pe = %s;" % ("(e %% %d) * %d + e / %d" % (nD, nC, nD) if self.swap_ket else "e"))
+        W("          const double v = ok ? tb[f] : 0.0;")
+        W("          if (v != 0.0) atomicAdd(&d.pairbuf[(size_t)pq * HFX_PAIRBUF_STRIDE + pe], v);")
+        W("        }")
+        W("        wave_sync();")
+        W("      }")
+        W("    }")
+        W("  }")
+        W("  __syncthreads();")
+        W("  if (curb >= 0) {")
+        W("    for (int e = threadIdx.x; e < NR * nbf; e += blockDim.x) {")
+        W("      const double v = srow[e];")
+        W("      if (v != 0.0) {")
+        W("        const int r = e / nbf, col = e - r * nbf;")
+        W("        atomicAdd(&X[(size_t)(r < %d ? rA + r : rB + (r - %d)) * nbf + col], v);" % (nA, nA))
+        W("      }")
+        W("    }")
+        W("  }")
+        W("}")
+        W("template __global__ void dig1_%s<double>(const DigestArgs);" % cls)
+        W("template __global__ void dig1_%s<float>(const DigestArgs);" % cls)
+
     def kernel(self, split):
         L = []
         W = L.append
@@ -1167,6 +1382,7 @@ class Gen:
         W("#include \"hfx_kernels.h\"")
         W("namespace hfx {")
         self.kernel_digest(W, None)
+        self.kernel_digest1(W)
         W("}  // namespace hfx")
         return "\n".join(L) + "\n"
 
@@ -1207,6 +1423,16 @@ SPLIT_COST = 200    # classes whose fused kernel would keep more doubles live ar
 # integral buffer of a lone context); it is faster for the other classes
 # (e.g. (dl|ll) FP64 39.9 -> 36.9 ms, (dd|ll) FP32 4.2 -> 2.8 ms).
 KPASS_TWO = {("dldl", "double"), ("ddll", "double"), ("dlll", "float")}
+# Block classes whose one-pass digestion (dig1_<class>) measured faster than
+# the multi-pass one on the C60 / 6-31G(d) builds, per precision of the list
+# (bit 0 FP64, bit 1 FP32); the library also requires long bra runs.  Measured
+# (ms, one-pass / multi-pass, kernels one at a time): FP64 build (dl|ll) 11.8 /
+# 18.8, (ll|ll) 10.2 / 11.7, (dd|ll) 2.6 / 5.0, (sd|ll) 0.59 / 0.84, (sd|dl)
+# 0.47 / 0.72, but (dl|dl) 12.8 / 10.1 and (dd|dl) 7.1 / 5.4 (their one-pass
+# kernels spill), (sl|ll) 3.7 / 3.3, (sl|dl) 3.8 / 3.0, (dd|dd) 1.4 / 0.7;
+# FP32 lists of the mixed build (dl|ll) 6.2 / 9.0, (dd|ll) 1.1 / 1.3, (dd|dd)
+# 0.20 / 0.36, but (ll|ll) 6.1 / 5.3, (dd|dl) 3.1 / 1.2, (sl|ll) 3.2 / 2.3.
+DIG1_USE = {"dlll": 3, "llll": 1, "ddll": 3, "sdll": 3, "sddl": 3, "dldl": 2, "dddd": 2, "sddd": 2}
 FP64_GROUP_ACC = 64    # block kernels: contracted accumulators per kernel, at most (FP64)
 FP32_GROUP_ACC = 128   # the same for the FP32 block kernels
 # per-class budgets picked by measurement (hfcode_amd/gen/autotune_groups.py);
@@ -1336,6 +1562,7 @@ def main(outdir):
             for k in range(table[c]["combos32"]):
                 D.append("template <typename R> __global__ void eri_%s_f%d(const ClassArgs);" % (c, k))
             D.append("template <bool JONLY, typename T, bool MULTI> __global__ void dig_%s(const DigestArgs);" % c)
+            D.append("template <typename T> __global__ void dig1_%s(const DigestArgs);" % c)
         else:
             D.append("template <typename R, bool JONLY> __global__ void eri_%s(const ClassArgs);" % c)
     D.append("struct ClassKernels {")
@@ -1347,6 +1574,8 @@ def main(outdir):
     D.append("  const void* dig; const void* digj; const void* digf;")
     D.append("  const void* digm; const void* digjm; const void* digfm;   // batch contexts (molecule-aware)")
     D.append("  float combo_cost[16]; float combo_cost32[16];   // relative cost of each combination kernel")
+    D.append("  const void* dig1; const void* dig1f;   // one-pass digestion (one molecule, J and K), FP64 / FP32 blocks")
+    D.append("  int dig1_use;   // bit 0 / 1: the one-pass digestion measured faster for FP64 / FP32 blocks")
     D.append("};")
     D.append("static const ClassKernels kClassKernels[%d] = {" % len(names))
     for c in names:
@@ -1365,11 +1594,12 @@ def main(outdir):
                    "(const void*)&dig_%s<false, float, true>" % (c, c, c))
             dg += ", {%s}, {%s}" % (", ".join("%.1ff" % x for x in t["combo_cost"]),
                                     ", ".join("%.1ff" % x for x in t["combo_cost32"]))
+            dg += ", (const void*)&dig1_%s<double>, (const void*)&dig1_%s<float>, %d" % (c, c, DIG1_USE.get(c, 0))
         else:
             fns = "(const void*)&eri_%s<double, false>, (const void*)&eri_%s<double, true>, (const void*)&eri_%s<float, false>" % (c, c, c)
             nc, cm, c64 = 0, "{0}", "{nullptr}"
             nc32, cm32, c32 = 0, "{0}", "{nullptr}"
-            dg = "nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, {0.0f}, {0.0f}"
+            dg = "nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, {0.0f}, {0.0f}, nullptr, nullptr, 0"
         D.append("  {%s, %d, %d, %d, %d, %d, %d, %d, {%d, %d, %d, %d}, %d, %d, %d, %s, %s, %d, %s, %s, %s}," % (
             fns, t["M"], int(t["all_prims"]), t["flops_prim"], t["flops_prim_fp32"], t["flops_braprim"],
             t["flops_quartet"], int(t["split"]), *t["dims"], int(t["swap_bra"]), int(t["swap_ket"]),

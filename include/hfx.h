@@ -37,6 +37,11 @@ extern "C" {
                                    digested), rounded down to a multiple of 256; 0 = largest the
                                    integral buffer holds (default).  A tuning knob: results do not
                                    depend on it. */
+#define HFX_OPT_DIGEST 5        /* block classes of a one-molecule context: 1 = digest the integral
+                                   blocks in one pass with the resident bra's rows of X and P in LDS
+                                   where that measured faster and the bra pairs' runs of quartets are
+                                   long (default); 2 = wherever its LDS fits; 0 = never (multi-pass
+                                   digestion).  Same sums, in another order. */
 
 /* pair types, in the order of the reference's pair lists (molprops.f90:122-131) */
 #define HFX_NPAIRTYPES 10       /* ss sp sd sl pp pd pl dd dl ll */
@@ -76,6 +81,7 @@ typedef struct hfx_stats {
   double ms_total;                           /* device time of the build, ms */
   double class_flops[HFX_NCLASSES][2];       /* algorithmic FLOPs per class kernel [fp64, fp32] */
   float class_ms[HFX_NCLASSES][2];           /* class kernel time, ms (HFX_OPT_PROFILE = 1) */
+  long long bra_runs[HFX_NCLASSES][2];       /* bra pairs with a surviving quartet [fp64, fp32] */
 } hfx_stats;
 
 /* Create a context on the current HIP device: copies the shells, builds the

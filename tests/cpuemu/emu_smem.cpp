@@ -256,7 +256,7 @@ size_t hfx_emu_arg_size(const void* fn) {
   static const std::unordered_set<const void*>* dig = [] {
     auto* s = new std::unordered_set<const void*>;
     for (const auto& k : hfx::kClassKernels)
-      for (const void* f : {k.dig, k.digj, k.digf, k.digm, k.digjm, k.digfm})
+      for (const void* f : {k.dig, k.digj, k.digf, k.digm, k.digjm, k.digfm, k.dig1, k.dig1f})
         if (f) s->insert(f);
     return s;
   }();

@@ -195,7 +195,7 @@ static bool run_screen_case(const ScreenCase& sc, unsigned seed, const char* nam
   const int nchunk = std::max(1, (maxseg + 63) / 64);
   const int cap = nbra * std::max(1, maxseg) + 1;
   std::vector<int2> list64(cap, make_int2(-1, -1)), list32(cap, make_int2(-1, -1));
-  std::vector<unsigned long long> masks((size_t)nbra * nchunk * 2 + 2, 0ull), stats(6, 0ull);
+  std::vector<unsigned long long> masks((size_t)nbra * nchunk * 2 + 2, 0ull), stats(8, 0ull);
   std::vector<int> counts(2, 0);
   ScreenArgs s;
   std::memset(&s, 0, sizeof(s));
@@ -216,7 +216,7 @@ static bool run_screen_case(const ScreenCase& sc, unsigned seed, const char* nam
   s.count64 = &counts[0];
   s.count32 = &counts[1];
   s.stats64 = &stats[0];
-  s.stats32 = &stats[3];
+  s.stats32 = &stats[4];
   s.pmax = pmax.data();
   s.masks = masks.data();
   s.thr = thr[0];
@@ -233,7 +233,17 @@ static bool run_screen_case(const ScreenCase& sc, unsigned seed, const char* nam
 
   CHECK(counts[0] == n64 && counts[1] == n32, "%s: counts %d %d, want %d %d", name, counts[0], counts[1], n64, n32);
   for (int k = 0; k < 3; ++k)
-    CHECK(stats[k] == st64[k] && stats[3 + k] == st32[k], "%s: statistics entry %d", name, k);
+    CHECK(stats[k] == st64[k] && stats[4 + k] == st32[k], "%s: statistics entry %d", name, k);
+  // bra runs: bras with at least one survivor in the list
+  unsigned long long runs64 = 0, runs32 = 0;
+  for (int i = 0; i < nbra; ++i) {
+    bool h64 = false, h32 = false;
+    for (const Quartet& q : want[i]) (q.fp32 ? h32 : h64) = true;
+    runs64 += h64;
+    runs32 += h32;
+  }
+  CHECK(stats[3] == runs64 && stats[7] == runs32, "%s: bra runs %llu %llu, want %llu %llu", name, stats[3], stats[7],
+        runs64, runs32);
   // every bra's quartets: one contiguous run per list, in the expected order
   for (int prec = 0; prec < 2; ++prec) {
     const std::vector<int2>& L = prec ? list32 : list64;

@@ -147,6 +147,30 @@ def test_chunked_builds(emu_fock):
     fb.close()
 
 
+@pytest.mark.parametrize("split,chunk", [(0.0, 0), (0.0, 256), (1e30, 0), (1e-5, 512)])
+def test_one_pass_digestion_equals_multi_pass(emu_fock, split, chunk):
+    """The block classes' one-pass digestion (resident bra's rows of X and P
+    in LDS, steps that never cross a bra pair) against the multi-pass one:
+    the same sums in another order, so FP64 builds agree to rounding and
+    all-FP32 builds to FP32 rounding of the per-quartet sums (a wrong row,
+    index or weight moves F by ~|e| |P| >= 1e-2).  Chunks of 256 and 512
+    quartets cut bra pairs' runs at chunk and workgroup boundaries."""
+    g = Golden("benzene_631gd")
+    z = np.zeros((g.nb, g.nb))
+    fb = emu_fock.FockBuilder(g.shells, g.pairs, nb=g.nb)
+    fb.set_mixed(split)
+    fb.set_chunk(chunk)
+    fb.set_one_pass_digest(2)   # every block class that fits, whatever the runs
+    one = fb.non_diag(z, g.P, 1e-10)
+    fb.set_one_pass_digest(False)
+    multi = fb.non_diag(z, g.P, 1e-10)
+    fb.close()
+    tol = 1e-12 if split == 0.0 else 1e-5
+    assert np.abs(one - multi).max() < tol
+    if split == 0.0:
+        assert np.abs(one - g["F_nd_1e-10"]).max() < TOL
+
+
 @pytest.mark.parametrize("name", ["water_sto3g", "water_def2svp"])
 def test_standalone_path_and_scf(emu_fock, name):
     """Own shells, one-electron matrices, Schwarz tables and SCF driver

@@ -82,6 +82,11 @@ def test_all_fp32_build_matches_fp64(emu_lanes, name):
     E.test_all_fp32_build_matches_fp64(emu_lanes, name)
 
 
+@pytest.mark.parametrize("split,chunk", [(0.0, 256), (1e30, 0)])
+def test_one_pass_digestion_equals_multi_pass(emu_lanes, split, chunk):
+    E.test_one_pass_digestion_equals_multi_pass(emu_lanes, split, chunk)
+
+
 def test_chunked_builds(emu_lanes):
     E.test_chunked_builds(emu_lanes)
 

@@ -37,6 +37,28 @@ def test_builds_match_golden(name):
         assert np.abs(fb.non_diag(z, g.P, thr) - g[key]).max() < TOL, thr
 
 
+@pytest.mark.parametrize("split,chunk", [(0.0, 0), (0.0, 256), (1e30, 0), (1e-5, 512)])
+def test_one_pass_digestion_equals_multi_pass(split, chunk):
+    """Block classes digested in one pass (resident bra's rows of X and P in
+    LDS) and in several: the same sums in another order (FP64 to rounding,
+    all-FP32 to FP32 rounding of the per-quartet sums); chunks of 256 and 512
+    quartets cut bra pairs' runs at chunk and workgroup boundaries."""
+    print("one-pass digestion, split %g chunk %d" % (split, chunk), flush=True)
+    g = Golden("benzene_631gd")
+    z = np.zeros((g.nb, g.nb))
+    fb = builder(g)
+    fb.set_mixed(split)
+    fb.set_chunk(chunk)
+    fb.set_one_pass_digest(2)   # every block class that fits, whatever the runs
+    one = fb.non_diag(z, g.P, 1e-10)
+    fb.set_one_pass_digest(False)
+    multi = fb.non_diag(z, g.P, 1e-10)
+    fb.close()
+    assert np.abs(one - multi).max() < (1e-12 if split == 0.0 else 1e-5)
+    if split == 0.0:
+        assert np.abs(one - g["F_nd_1e-10"]).max() < TOL
+
+
 @pytest.mark.parametrize("name", ["water_def2svp", "benzene_631gd"])
 def test_j_only_matches_oracle(name):
     import oracle
