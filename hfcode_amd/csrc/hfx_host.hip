@@ -362,6 +362,8 @@ struct hfx_ctx {
   int* d_aseg[HFX_NPAIRTYPES] = {nullptr};       // first active position per molecule (nmol + 1)
   unsigned long long* d_masks = nullptr;       // screening survivor bits of one class
   size_t masks_words = 0;
+  unsigned long long* d_masks_side = nullptr;  // the same for the screens on the side streams (kSide x masks_words)
+  size_t masks_side_words = 0;
   int act_sorted[HFX_NPAIRTYPES] = {0};         // active Schwarz values non-increasing
   double* d_schw_out = nullptr;   // per global pair
   int* d_act[HFX_NPAIRTYPES] = {nullptr};        // active global pair indices (rank order)
@@ -851,8 +853,47 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
     hipLaunchKernelGGL(k_to_f32, dim3(grid_for((long long)nb2, 256, 2048)), dim3(256), 0, c->stream, dP, c->d_P32, nb2);
     HIPOK(hipGetLastError());
   }
-  for (auto& cp : c->classes) {
-    if (cp.cap <= 0) continue;
+  // The classes' screens are independent but each leaves most of the GPU idle
+  // (one wave per bra position): with one live context and launches sized
+  // on the host, they run side by side on the context's stream and the side
+  // streams (their own survivor-mask buffers), costliest first (bra x ket
+  // candidates) to the least-loaded stream, and join before the counts are read.
+  const bool conc_screen = live_contexts.load() == 1 && !c->device_chunks && !c->capturing;
+  hipStream_t sst[hfx_ctx::kSide + 1] = {c->stream};
+  unsigned long long* smask[hfx_ctx::kSide + 1] = {c->d_masks};
+  int nsst = 1;
+  if (conc_screen) {
+    int r = ensure_side_streams(c);
+    if (r) return r;
+    if (c->masks_side_words < c->masks_words * hfx_ctx::kSide) {
+      HIPOK(sync_all_streams(c));   // an earlier build's screens may still read the old buffer
+      dfree(c->d_masks_side);
+      c->masks_side_words = 0;
+      if (dalloc(&c->d_masks_side, c->masks_words * hfx_ctx::kSide)) return HFX_E_NOMEM;
+      c->masks_side_words = c->masks_words * hfx_ctx::kSide;
+    }
+    HIPOK(hipEventRecord(c->fork, c->stream));
+    for (int k = 0; k < hfx_ctx::kSide; ++k) {
+      HIPOK(hipStreamWaitEvent(c->side[k], c->fork, 0));
+      sst[k + 1] = c->side[k];
+      smask[k + 1] = c->d_masks_side + (size_t)k * c->masks_words;
+    }
+    nsst = hfx_ctx::kSide + 1;
+  }
+  std::vector<int> sorder;
+  for (int i = 0; i < (int)c->classes.size(); ++i)
+    if (c->classes[i].cap > 0) sorder.push_back(i);
+  std::stable_sort(sorder.begin(), sorder.end(), [&](int x, int y) {
+    return (double)c->act_len[c->classes[x].t1] * c->act_len[c->classes[x].t2] >
+           (double)c->act_len[c->classes[y].t1] * c->act_len[c->classes[y].t2];
+  });
+  double sload[hfx_ctx::kSide + 1] = {};
+  for (int ci : sorder) {
+    auto& cp = c->classes[ci];
+    int si = 0;
+    for (int k = 1; k < nsst; ++k)
+      if (sload[k] < sload[si]) si = k;
+    sload[si] += (double)c->act_len[cp.t1] * c->act_len[cp.t2];
     ScreenArgs s;
     memset(&s, 0, sizeof(s));
     s.bra = c->d_act[cp.t1];
@@ -883,12 +924,16 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
     s.nsh = c->nsp;
     s.all_prims = kClassKernels[cp.cls].all_prims;
     s.pmax = c->d_pmax;
-    s.masks = c->d_masks;
+    s.masks = smask[si];
     s.nchunk = c->nchunk_max[cp.t2];
     s.ket_sorted = c->act_sorted[cp.t2];
     // one wave per bra position
-    hipLaunchKernelGGL(k_screen, dim3(grid_for(s.nbra, 256 / HFX_WAVE, c->ncu * 16)), dim3(256), 0, c->stream, s);
+    hipLaunchKernelGGL(k_screen, dim3(grid_for(s.nbra, 256 / HFX_WAVE, c->ncu * 16)), dim3(256), 0, sst[si], s);
     HIPOK(hipGetLastError());
+  }
+  for (int k = 1; k < nsst; ++k) {
+    HIPOK(hipEventRecord(c->join[k - 1], sst[k]));
+    HIPOK(hipStreamWaitEvent(c->stream, c->join[k - 1], 0));
   }
   // surviving-quartet counts of the block classes decide their chunking
   int hcounts[2 * HFX_NCLASSES] = {0};
@@ -1336,6 +1381,7 @@ void hfx_ctx_destroy(hfx_ctx* c) {
   dfree(c->d_onee_items); dfree(c->d_sh_type); dfree(c->d_sh_np); dfree(c->d_sh_ao);
   dfree(c->d_sh_xyz); dfree(c->d_sh_z); dfree(c->d_sh_c); dfree(c->d_sh_cl);
   dfree(c->d_counts); dfree(c->d_stats); dfree(c->d_eblk); dfree(c->d_vmax); dfree(c->d_pmax); dfree(c->d_masks);
+  dfree(c->d_masks_side);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   for (int p = 0; p < 2; ++p)

@@ -1142,10 +1142,6 @@ class Gen:
 
         orow = "o" if a_outer else "%d + o" % nA
         irow = (lambda i: "%d" % (nA + i)) if a_outer else (lambda i: "%d" % i)
-        arow = "o" if a_outer else None   # row of a in srow / prow
-
-        def pin(names):
-            W("        " + " ".join("HFX_PIN(%s);" % n for n in names) + " HFX_LOAD_FENCE();")
 
         W("template <typename T>")
         W("__global__ void __launch_bounds__(DIG1_NT) dig1_%s(const DigestArgs d) {" % cls)
@@ -1219,71 +1215,94 @@ class Gen:
         W("    const int aoC = pk.ao%s, aoD = pk.ao%s;" % (oc, od))
         W("    const double w = act ? ((pb.sha == pb.shb) ? 0.5 : 1.0) * ((shC == shD) ? 0.5 : 1.0) : 0.0;")
         W("    const double wj = 2.0 * w, wk = -0.5 * w;")
-        for c in range(nC):
-            for dd in range(nD):
-                W("    const T Pcd_%d_%d = PA[(size_t)(aoC + %d) * nbf + aoD + %d];" % (c, dd, c, dd))
         W("    " + " ".join("T Jcd_%d = 0;" % k for k in range(ncd)))
-        for i in range(n_i):
-            W("    " + " ".join("T Kic_%d_%d = 0;" % (i, c) for c in range(nC)) + " "
-              + " ".join("T Kid_%d_%d = 0;" % (i, dd) for dd in range(nD)))
-        W("    #pragma unroll 1")
-        W("    for (int o = 0; o < %d; ++o) {" % n_o)
-        W("      const T* bp = blk + (size_t)o * %d; HFX_PIN(bp);" % o_stride)
-        W("      const T* po = prow + (size_t)(%s) * nbf;   // P row of the outer index" % orow)
-        for dd in range(nD):
-            W("      const T Pod_%d = po[aoD + %d];" % (dd, dd))
-        for c in range(nC):
-            W("      const T Poc_%d = po[aoC + %d];" % (c, c))
-        W("      " + " ".join("T Koc_%d = 0;" % c for c in range(nC)) + " " + " ".join("T Kod_%d = 0;" % dd for dd in range(nD)))
-        for i in range(n_i):
-            W("      {")
-            W("        const T* pi = prow + (size_t)%s * nbf;   // P row of inner index %d" % (irow(i), i))
-            for dd in range(nD):
-                W("        const T Pid_%d = pi[aoD + %d];" % (dd, dd))
-            for c in range(nC):
-                W("        const T Pic_%d = pi[aoC + %d];" % (c, c))
-            # P_ab of (a, b) = (o, i) or (i, o): row a of A, column b of B
-            if a_outer:
-                W("        const T pab = prow[(size_t)o * nbf + rB + %d];" % i)
-            else:
-                W("        const T pab = prow[(size_t)%d * nbf + rB + o];" % i)
-            W("        T jab = 0;")
+
+        def core(ki_slice):
+            """The contractions of one quartet.  ki_slice: X_ic, X_id go to the
+            LDS rows after every (o, i) slice (fewer registers: the FP64 form
+            holds no more than the running slice's sums); else they are summed
+            over o in registers and flushed once (the FP32 form)."""
             for c in range(nC):
                 for dd in range(nD):
-                    k = c * nD + dd
-                    W("        { const T v = bp[%d]; jab = fma(v, Pcd_%d_%d, jab); Jcd_%d = fma(v, pab, Jcd_%d);"
-                      " Koc_%d = fma(v, Pid_%d, Koc_%d); Kod_%d = fma(v, Pic_%d, Kod_%d);"
-                      " Kic_%d_%d = fma(v, Pod_%d, Kic_%d_%d); Kid_%d_%d = fma(v, Poc_%d, Kid_%d_%d); }" % (
-                          lin_i(i, c, dd), c, dd, k, k, c, dd, c, dd, c, dd,
-                          i, c, dd, i, c, i, dd, c, i, dd))
-            # X_ab of this (a, b): summed over the wave (every lane has the resident bra)
-            W("        double r = wj * (double)jab;")
-            W("        for (int s = HFX_WAVE / 2; s > 0; s >>= 1) r += __shfl_xor(r, s);")
-            if a_outer:
-                W("        if (lane == 0 && r != 0.0) atomicAdd(&srow[(size_t)o * nbf + rB + %d], r);" % i)
-            else:
-                W("        if (lane == 0 && r != 0.0) atomicAdd(&srow[(size_t)%d * nbf + rB + o], r);" % i)
-            pin(["Jcd_%d" % k for k in range(ncd)] + ["Koc_%d" % c for c in range(nC)]
-                + ["Kod_%d" % dd for dd in range(nD)] + ["Kic_%d_%d" % (i, c) for c in range(nC)]
-                + ["Kid_%d_%d" % (i, dd) for dd in range(nD)])
-            W("      }")
-        W("      if (act) {")
-        W("        double* so = srow + (size_t)(%s) * nbf;" % orow)
-        for c in range(nC):
-            W("        atomicAdd(&so[aoC + %d], wk * (double)Koc_%d);" % (c, c))
-        for dd in range(nD):
-            W("        atomicAdd(&so[aoD + %d], wk * (double)Kod_%d);" % (dd, dd))
-        W("      }")
-        W("    }")
-        W("    if (act) {")
-        for i in range(n_i):
-            W("      {")
-            W("        double* si = srow + (size_t)%s * nbf;" % irow(i))
-            for c in range(nC):
-                W("        atomicAdd(&si[aoC + %d], wk * (double)Kic_%d_%d);" % (c, i, c))
+                    W("      const T Pcd_%d_%d = PA[(size_t)(aoC + %d) * nbf + aoD + %d];" % (c, dd, c, dd))
+            if not ki_slice:
+                for i in range(n_i):
+                    W("      " + " ".join("T Kic_%d_%d = 0;" % (i, c) for c in range(nC)) + " "
+                      + " ".join("T Kid_%d_%d = 0;" % (i, dd) for dd in range(nD)))
+            W("      #pragma unroll 1")
+            W("      for (int o = 0; o < %d; ++o) {" % n_o)
+            W("        const T* bp = blk + (size_t)o * %d; HFX_PIN(bp);" % o_stride)
+            W("        const T* po = prow + (size_t)(%s) * nbf;   // P row of the outer index" % orow)
             for dd in range(nD):
-                W("        atomicAdd(&si[aoD + %d], wk * (double)Kid_%d_%d);" % (dd, i, dd))
+                W("        const T Pod_%d = po[aoD + %d];" % (dd, dd))
+            for c in range(nC):
+                W("        const T Poc_%d = po[aoC + %d];" % (c, c))
+            W("        " + " ".join("T Koc_%d = 0;" % c for c in range(nC)) + " "
+              + " ".join("T Kod_%d = 0;" % dd for dd in range(nD)))
+            for i in range(n_i):
+                W("        {")
+                W("          const T* pi = prow + (size_t)%s * nbf;   // P row of inner index %d" % (irow(i), i))
+                for dd in range(nD):
+                    W("          const T Pid_%d = pi[aoD + %d];" % (dd, dd))
+                for c in range(nC):
+                    W("          const T Pic_%d = pi[aoC + %d];" % (c, c))
+                # P_ab of (a, b) = (o, i) or (i, o): row a of A, column b of B
+                if a_outer:
+                    W("          const T pab = prow[(size_t)o * nbf + rB + %d];" % i)
+                else:
+                    W("          const T pab = prow[(size_t)%d * nbf + rB + o];" % i)
+                W("          T jab = 0;")
+                if ki_slice:
+                    W("          " + " ".join("T Kic_%d_%d = 0;" % (i, c) for c in range(nC)) + " "
+                      + " ".join("T Kid_%d_%d = 0;" % (i, dd) for dd in range(nD)))
+                for c in range(nC):
+                    for dd in range(nD):
+                        k = c * nD + dd
+                        W("          { const T v = bp[%d]; jab = fma(v, Pcd_%d_%d, jab); Jcd_%d = fma(v, pab, Jcd_%d);"
+                          " Koc_%d = fma(v, Pid_%d, Koc_%d); Kod_%d = fma(v, Pic_%d, Kod_%d);"
+                          " Kic_%d_%d = fma(v, Pod_%d, Kic_%d_%d); Kid_%d_%d = fma(v, Poc_%d, Kid_%d_%d); }" % (
+                              lin_i(i, c, dd), c, dd, k, k, c, dd, c, dd, c, dd,
+                              i, c, dd, i, c, i, dd, c, i, dd))
+                # X_ab of this (a, b): summed over the wave (every lane has the resident bra)
+                W("          double r = wj * (double)jab;")
+                W("          for (int s = HFX_WAVE / 2; s > 0; s >>= 1) r += __shfl_xor(r, s);")
+                if a_outer:
+                    W("          if (lane == 0 && r != 0.0) atomicAdd(&srow[(size_t)o * nbf + rB + %d], r);" % i)
+                else:
+                    W("          if (lane == 0 && r != 0.0) atomicAdd(&srow[(size_t)%d * nbf + rB + o], r);" % i)
+                held = [] if ki_slice else (["Kic_%d_%d" % (i, c) for c in range(nC)]
+                                            + ["Kid_%d_%d" % (i, dd) for dd in range(nD)])
+                W("          " + " ".join("HFX_PIN(%s);" % n for n in ["Jcd_%d" % k for k in range(ncd)]
+                                          + ["Koc_%d" % c for c in range(nC)] + ["Kod_%d" % dd for dd in range(nD)]
+                                          + held) + " HFX_LOAD_FENCE();")
+                if ki_slice:
+                    kflush_i(i, "          ")
+                W("        }")
+            W("        if (act) {")
+            W("          double* so = srow + (size_t)(%s) * nbf;" % orow)
+            for c in range(nC):
+                W("          atomicAdd(&so[aoC + %d], wk * (double)Koc_%d);" % (c, c))
+            for dd in range(nD):
+                W("          atomicAdd(&so[aoD + %d], wk * (double)Kod_%d);" % (dd, dd))
+            W("        }")
             W("      }")
+            if not ki_slice:
+                for i in range(n_i):
+                    kflush_i(i, "      ")
+
+        def kflush_i(i, ind):
+            W(ind + "if (act) {")
+            W(ind + "  double* si = srow + (size_t)%s * nbf;" % irow(i))
+            for c in range(nC):
+                W(ind + "  atomicAdd(&si[aoC + %d], wk * (double)Kic_%d_%d);" % (c, i, c))
+            for dd in range(nD):
+                W(ind + "  atomicAdd(&si[aoD + %d], wk * (double)Kid_%d_%d);" % (dd, i, dd))
+            W(ind + "}")
+
+        W("    if constexpr (sizeof(T) == sizeof(double)) {")
+        core(True)
+        W("    } else {")
+        core(False)
         W("    }")
         # X_cd through the wave's LDS staging into the ket pair's slot of the pair buffer
         W("    {")
@@ -1425,14 +1444,14 @@ SPLIT_COST = 200    # classes whose fused kernel would keep more doubles live ar
 KPASS_TWO = {("dldl", "double"), ("ddll", "double"), ("dlll", "float")}
 # Block classes whose one-pass digestion (dig1_<class>) measured faster than
 # the multi-pass one on the C60 / 6-31G(d) builds, per precision of the list
-# (bit 0 FP64, bit 1 FP32); the library also requires long bra runs.  Measured
-# (ms, one-pass / multi-pass, kernels one at a time): FP64 build (dl|ll) 11.8 /
-# 18.8, (ll|ll) 10.2 / 11.7, (dd|ll) 2.6 / 5.0, (sd|ll) 0.59 / 0.84, (sd|dl)
-# 0.47 / 0.72, but (dl|dl) 12.8 / 10.1 and (dd|dl) 7.1 / 5.4 (their one-pass
-# kernels spill), (sl|ll) 3.7 / 3.3, (sl|dl) 3.8 / 3.0, (dd|dd) 1.4 / 0.7;
-# FP32 lists of the mixed build (dl|ll) 6.2 / 9.0, (dd|ll) 1.1 / 1.3, (dd|dd)
-# 0.20 / 0.36, but (ll|ll) 6.1 / 5.3, (dd|dl) 3.1 / 1.2, (sl|ll) 3.2 / 2.3.
-DIG1_USE = {"dlll": 3, "llll": 1, "ddll": 3, "sdll": 3, "sddl": 3, "dldl": 2, "dddd": 2, "sddd": 2}
+# (bit 0 FP64, bit 1 FP32); the library also requires long bra runs.  Class
+# spans, ms, one-pass / multi-pass (FP64 form flushing X_ic, X_id per slice):
+# FP64 build (dl|ll) 26.9 / 34.5, (dl|dl) 26.4 / 31.8, (dd|dl) 21.5 / 22.8,
+# (ll|ll) 19.4 / 21.8, (dd|ll) 12.2 / 14.7, but (sl|ll) 8.8 / 8.5, (sl|dl)
+# 7.3 / 6.7, (dd|dd) 3.4 / 3.2; FP32 lists of the mixed build (dl|ll) 12.3 /
+# 16.7, (ll|ll) 9.5 / 9.8, (dd|dd) 1.13 / 1.27, but (dd|dl) 6.8 / 4.8, (sl|ll)
+# 5.3 / 4.9, (sl|dl) 4.7 / 3.7.
+DIG1_USE = {"dlll": 3, "llll": 3, "ddll": 3, "sdll": 3, "sddl": 3, "dldl": 1, "dddl": 1, "dddd": 2, "sddd": 2}
 FP64_GROUP_ACC = 64    # block kernels: contracted accumulators per kernel, at most (FP64)
 FP32_GROUP_ACC = 128   # the same for the FP32 block kernels
 # per-class budgets picked by measurement (hfcode_amd/gen/autotune_groups.py);
