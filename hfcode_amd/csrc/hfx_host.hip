@@ -555,7 +555,6 @@ const long long kEblkDoubles = (1ll << 20) * 625;       // integral buffer of th
 const long long kEblkDoublesAlone = (1ll << 21) * 625;  // ... or this when it grows in the one live context (10.5 GB)
 const long long kMaxChunk = 1 << 22;                      // quartets per chunk, at most
 const int kMaxCombo = 16;                            // combination kernels per class, at most
-const long long kDig1MinRun = 1024;   // one-pass digestion: surviving quartets per bra run, at least (mean)
 
 const long long kConcurrentChunk = 1 << 17;   // quartets of a chunk from which its combinations run side by side
 std::atomic<int> live_contexts{0};             // contexts created and not yet destroyed, process-wide
@@ -657,7 +656,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   // 2 takes it wherever it fits
   const bool one_ok = c->digest1 == 2 ||
                       (c->digest1 == 1 && (k.dig1_use >> (prec == 1 ? 1 : 0) & 1) &&
-                       (runs <= 0 || n >= kDig1MinRun * runs));
+                       (runs <= 0 || n >= (long long)k.dig1_min_run * runs));
   const bool one_pass = one_ok && !a.diag && !jonly && c->nmol == 1 && dump == nullptr &&
                         a.schw_out == nullptr && k.dig1 != nullptr && one_shm <= c->max_shm;
   const int ncombo = prec == 1 ? k.ncombo32 : k.ncombo;

@@ -1217,11 +1217,17 @@ class Gen:
         W("    const double wj = 2.0 * w, wk = -0.5 * w;")
         W("    " + " ".join("T Jcd_%d = 0;" % k for k in range(ncd)))
 
-        def core(ki_slice):
+        def core(ki_slice, prefetch):
             """The contractions of one quartet.  ki_slice: X_ic, X_id go to the
             LDS rows after every (o, i) slice (fewer registers: the FP64 form
             holds no more than the running slice's sums); else they are summed
-            over o in registers and flushed once (the FP32 form)."""
+            over o in registers and flushed once (the FP32 form).  prefetch:
+            the loads of slice i + 1 are issued before slice i is contracted
+            (two slices of the block in flight per wave)."""
+
+            def sload(i):
+                W("          " + " ".join("s%d_%d = bp[%d];" % (i, c * nD + dd, lin_i(i, c, dd))
+                                          for c in range(nC) for dd in range(nD)))
             for c in range(nC):
                 for dd in range(nD):
                     W("      const T Pcd_%d_%d = PA[(size_t)(aoC + %d) * nbf + aoD + %d];" % (c, dd, c, dd))
@@ -1239,8 +1245,15 @@ class Gen:
                 W("        const T Poc_%d = po[aoC + %d];" % (c, c))
             W("        " + " ".join("T Koc_%d = 0;" % c for c in range(nC)) + " "
               + " ".join("T Kod_%d = 0;" % dd for dd in range(nD)))
+            if prefetch:
+                W("        " + " ".join("T s0_%d = bp[%d];" % (c * nD + dd, lin_i(0, c, dd))
+                                        for c in range(nC) for dd in range(nD)))
+                for i in range(1, n_i):
+                    W("        " + " ".join("T s%d_%d;" % (i, k) for k in range(ncd)))
             for i in range(n_i):
                 W("        {")
+                if prefetch and i + 1 < n_i:
+                    sload(i + 1)
                 W("          const T* pi = prow + (size_t)%s * nbf;   // P row of inner index %d" % (irow(i), i))
                 for dd in range(nD):
                     W("          const T Pid_%d = pi[aoD + %d];" % (dd, dd))
@@ -1258,10 +1271,11 @@ class Gen:
                 for c in range(nC):
                     for dd in range(nD):
                         k = c * nD + dd
-                        W("          { const T v = bp[%d]; jab = fma(v, Pcd_%d_%d, jab); Jcd_%d = fma(v, pab, Jcd_%d);"
+                        src = ("s%d_%d" % (i, k)) if prefetch else ("bp[%d]" % lin_i(i, c, dd))
+                        W("          { const T v = %s; jab = fma(v, Pcd_%d_%d, jab); Jcd_%d = fma(v, pab, Jcd_%d);"
                           " Koc_%d = fma(v, Pid_%d, Koc_%d); Kod_%d = fma(v, Pic_%d, Kod_%d);"
                           " Kic_%d_%d = fma(v, Pod_%d, Kic_%d_%d); Kid_%d_%d = fma(v, Poc_%d, Kid_%d_%d); }" % (
-                              lin_i(i, c, dd), c, dd, k, k, c, dd, c, dd, c, dd,
+                              src, c, dd, k, k, c, dd, c, dd, c, dd,
                               i, c, dd, i, c, i, dd, c, i, dd))
                 # X_ab of this (a, b): summed over the wave (every lane has the resident bra)
                 W("          double r = wj * (double)jab;")
@@ -1300,9 +1314,9 @@ class Gen:
             W(ind + "}")
 
         W("    if constexpr (sizeof(T) == sizeof(double)) {")
-        core(True)
+        core(True, cls in DIG1_PREFETCH64)
         W("    } else {")
-        core(False)
+        core(False, DIG1_PREFETCH32)
         W("    }")
         # X_cd through the wave's LDS staging into the ket pair's slot of the pair buffer
         W("    {")
@@ -1451,7 +1465,19 @@ KPASS_TWO = {("dldl", "double"), ("ddll", "double"), ("dlll", "float")}
 # 7.3 / 6.7, (dd|dd) 3.4 / 3.2; FP32 lists of the mixed build (dl|ll) 12.3 /
 # 16.7, (ll|ll) 9.5 / 9.8, (dd|dd) 1.13 / 1.27, but (dd|dl) 6.8 / 4.8, (sl|ll)
 # 5.3 / 4.9, (sl|dl) 4.7 / 3.7.
+# The FP64 list of the mixed build has shorter runs (mean quartets per bra
+# pair: (dl|ll) 1431, (dd|dl) 831, (ll|ll) 803, (dl|dl) 385): there (dd|dl)
+# still gains (4.4 / 5.2 ms) while (ll|ll) loses (7.7 / 7.1), so the minimum
+# mean run is per class (DIG1_MIN_RUN, default 1024 quartets).
 DIG1_USE = {"dlll": 3, "llll": 3, "ddll": 3, "sdll": 3, "sddl": 3, "dldl": 1, "dddl": 1, "dddd": 2, "sddd": 2}
+DIG1_MIN_RUN = {"dddl": 512}
+# one-pass digestion: next slice's loads issued before the current slice's
+# contractions, for the FP32 form and the FP64 form of the classes listed
+# (measured, C60 builds, ms: FP32 (dl|ll) 12.3 -> 11.9, (ll|ll) 9.3 -> 9.1,
+# (dd|ll) 2.39 -> 2.27; FP64 (ll|ll) 19.2 -> 18.8; the FP64 forms of (dl|dl)
+# and (dd|dl) would spill 150 bytes per lane)
+DIG1_PREFETCH32 = True
+DIG1_PREFETCH64 = {"dlll", "llll"}
 FP64_GROUP_ACC = 64    # block kernels: contracted accumulators per kernel, at most (FP64)
 FP32_GROUP_ACC = 128   # the same for the FP32 block kernels
 # per-class budgets picked by measurement (hfcode_amd/gen/autotune_groups.py);
@@ -1595,6 +1621,7 @@ def main(outdir):
     D.append("  float combo_cost[16]; float combo_cost32[16];   // relative cost of each combination kernel")
     D.append("  const void* dig1; const void* dig1f;   // one-pass digestion (one molecule, J and K), FP64 / FP32 blocks")
     D.append("  int dig1_use;   // bit 0 / 1: the one-pass digestion measured faster for FP64 / FP32 blocks")
+    D.append("  int dig1_min_run;   // ... when a list's mean run of quartets per bra pair is at least this")
     D.append("};")
     D.append("static const ClassKernels kClassKernels[%d] = {" % len(names))
     for c in names:
@@ -1613,12 +1640,13 @@ def main(outdir):
                    "(const void*)&dig_%s<false, float, true>" % (c, c, c))
             dg += ", {%s}, {%s}" % (", ".join("%.1ff" % x for x in t["combo_cost"]),
                                     ", ".join("%.1ff" % x for x in t["combo_cost32"]))
-            dg += ", (const void*)&dig1_%s<double>, (const void*)&dig1_%s<float>, %d" % (c, c, DIG1_USE.get(c, 0))
+            dg += ", (const void*)&dig1_%s<double>, (const void*)&dig1_%s<float>, %d, %d" % (
+                c, c, DIG1_USE.get(c, 0), DIG1_MIN_RUN.get(c, 1024))
         else:
             fns = "(const void*)&eri_%s<double, false>, (const void*)&eri_%s<double, true>, (const void*)&eri_%s<float, false>" % (c, c, c)
             nc, cm, c64 = 0, "{0}", "{nullptr}"
             nc32, cm32, c32 = 0, "{0}", "{nullptr}"
-            dg = "nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, {0.0f}, {0.0f}, nullptr, nullptr, 0"
+            dg = "nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, {0.0f}, {0.0f}, nullptr, nullptr, 0, 0"
         D.append("  {%s, %d, %d, %d, %d, %d, %d, %d, {%d, %d, %d, %d}, %d, %d, %d, %s, %s, %d, %s, %s, %s}," % (
             fns, t["M"], int(t["all_prims"]), t["flops_prim"], t["flops_prim_fp32"], t["flops_braprim"],
             t["flops_quartet"], int(t["split"]), *t["dims"], int(t["swap_bra"]), int(t["swap_ket"]),
