@@ -73,6 +73,8 @@ struct ClassArgs {
   int rows_offset;         // fused kernels: byte offset of those rows in the dynamic LDS
   double* pairbuf;         // fused kernels: J_cd per ket pair (HFX_PAIRBUF_STRIDE doubles per pair)
   const double* thr_mol;   // block test threshold per molecule (device), or null: thr for every quartet
+  double* cbuf;            // block kernels: contracted values of split combinations, [value][quartet - q0]
+                           // (stored as R, vstride apart)
 };
 
 // Resident bra pair for a step [lo, hi) of a bra-contiguous quartet list

@@ -221,6 +221,49 @@ static void* scratch_address(int dev) {
   return g_scratch_ptr[dev];
 }
 
+// Dispatches that need more scratch memory (spilled registers) than the
+// device's scratch-limit threshold get a use-once scratch allocation per
+// dispatch; below it the runtime keeps the allocation on the queue.  The
+// largest combination kernels spill kilobytes per lane, so the threshold is
+// raised, once per device, to what the library's kernels need at full
+// occupancy (at most the device's maximum).
+static void raise_scratch_limit(int dev, const hipDeviceProp_t& prop) {
+  static std::mutex mu;
+  static std::vector<char> done;
+  std::lock_guard<std::mutex> lk(mu);
+  if (dev < 0) return;
+  if ((size_t)dev >= done.size()) done.resize(dev + 1, 0);
+  if (done[dev]) return;
+  done[dev] = 1;
+  size_t lane = 0;
+  auto take = [&](const void* fn) {
+    hipFuncAttributes at;
+    if (fn && hipFuncGetAttributes(&at, fn) == hipSuccess) lane = std::max(lane, (size_t)at.localSizeBytes);
+    else (void)hipGetLastError();
+  };
+  for (const ClassKernels& k : kClassKernels) {
+    for (const void* fn : {k.f64, k.f64j, k.f32, k.dig, k.digj, k.digf, k.digm, k.digjm, k.digfm, k.dig1, k.dig1f})
+      take(fn);
+    for (int i = 0; i < k.ncombo; ++i) {
+      take(k.c64[i]);
+      for (int j = 0; j < k.npre[i]; ++j) take(k.pre64[i][j]);
+    }
+    for (int i = 0; i < k.ncombo32; ++i) {
+      take(k.c32[i]);
+      for (int j = 0; j < k.npre32[i]; ++j) take(k.pre32[i][j]);
+    }
+  }
+  size_t mx = 0, cur = 0;
+  if (lane == 0 || hipDeviceGetLimit(&mx, hipExtLimitScratchMax) != hipSuccess ||
+      hipDeviceGetLimit(&cur, hipExtLimitScratchCurrent) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  const size_t want = std::min(mx, lane * HFX_WAVE * (size_t)prop.multiProcessorCount *
+                                       (size_t)std::max(1, prop.maxThreadsPerMultiProcessor / HFX_WAVE));
+  if (want > cur && hipDeviceSetLimit(hipExtLimitScratchCurrent, want) != hipSuccess) (void)hipGetLastError();
+}
+
 std::shared_mutex& hfx::capture_mutex() {
   static std::shared_mutex* m = new std::shared_mutex;   // lives until the process ends
   return *m;
@@ -323,6 +366,8 @@ struct hfx_ctx {
   int profile = 0;
   double* d_eblk = nullptr;     // integral blocks of the block class kernels
   double* d_vmax = nullptr;     // their partial max|e|, kMaxCombo rows of vmax_rows
+  double* d_cbuf = nullptr;     // contracted values of split combinations (pre-kernels to the last kernel)
+  long long cbuf_doubles = 0;   // capacity of d_cbuf
   long long eblk_doubles = 0;   // capacity of d_eblk
   long long vmax_rows = 0;      // quartets per row of d_vmax
   size_t max_shm = 0;           // LDS bytes a workgroup may use
@@ -627,6 +672,27 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     int r = ensure_side_streams(c);
     if (r) return r;
   }
+  // Split combinations: their pre-kernels store cbn doubles per quartet of
+  // the chunk (floats: half), per half of the pipeline; the buffer has the
+  // integral buffer's budget, and chunks shrink to fit it.
+  const long long cbn = prec == 1 ? (k.cbuf32 + 1) / 2 : k.cbuf64;
+  if (cbn > 0) {
+    const long long halves = pipelined ? 2 : 1;
+    const long long cmax = limit / (halves * cbn) / HFX_NT * HFX_NT;
+    if (cmax < HFX_NT) return HFX_E_NOMEM;
+    chunk = std::min(chunk, cmax);
+    const long long need = halves * cbn * chunk;
+    if (need > c->cbuf_doubles) {
+      if (c->capturing) return HFX_E_STATE;   // sized by an uncaptured build first
+      const long long cap = std::min(limit, std::max(need, 2 * c->cbuf_doubles));
+      HIPOK(sync_all_streams(c));
+      drop_graphs(c);
+      dfree(c->d_cbuf);
+      c->cbuf_doubles = 0;
+      if (dalloc(&c->d_cbuf, (size_t)cap)) return HFX_E_NOMEM;
+      c->cbuf_doubles = cap;
+    }
+  }
   // every chunk's blocks and partial maxima stay inside its half (or the
   // whole buffer) of the integral buffer and of vmax
   if (chunk < HFX_NT || chunk * ne > (pipelined ? half_e : c->eblk_doubles) ||
@@ -675,6 +741,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     ClassArgs b = a;
     b.eblk = eblk;
     b.vmax = vmax;
+    b.cbuf = cbn > 0 ? c->d_cbuf + h * cbn * chunk : nullptr;
     b.vstride = (int)chunk;
     b.q0 = (int)q0;
     b.q1 = (int)q1;
@@ -714,7 +781,12 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
       load[sb] += cost[ic];
       hipStream_t st = pipelined ? c->side[sb] : (sb == 0 ? c->stream : c->side[sb - 1]);
       const int per_cu = c->cls_per_cu[cls] > 0 ? c->cls_per_cu[cls] : 8;   // tuned launch width
-      HIPOK(hipLaunchKernel(fn, dim3(grid_for(q1 - q0, HFX_NT, c->ncu * per_cu)), dim3(HFX_NT), args, shm, st));
+      const dim3 grid(grid_for(q1 - q0, HFX_NT, c->ncu * per_cu));
+      // a split combination's pre-kernels first, on the same stream
+      const int npre = prec == 1 ? k.npre32[ic] : k.npre[ic];
+      for (int jp = 0; jp < npre; ++jp)
+        HIPOK(hipLaunchKernel(prec == 1 ? k.pre32[ic][jp] : k.pre64[ic][jp], grid, dim3(HFX_NT), args, shm, st));
+      HIPOK(hipLaunchKernel(fn, grid, dim3(HFX_NT), args, shm, st));
     }
     for (int s = 0; s < nside; ++s) {
       HIPOK(hipEventRecord(c->join[s], c->side[s]));
@@ -1317,6 +1389,7 @@ int hfx_batch_create(const hfx_mol_desc* descs, int nmol, hfx_ctx** out) {
   c->ncu = prop.multiProcessorCount;
   c->max_shm = prop.sharedMemPerBlock;
   c->scratch = scratch_address(c->dev);   // set-up of hfx_ctx_onee_noalloc, once per device
+  raise_scratch_limit(c->dev, prop);      // spilling kernels keep their scratch on the queue
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(HFX_E_HIP);
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(HFX_E_HIP);
   std::vector<ShellRec> sr(c->nsh);
@@ -1380,7 +1453,7 @@ void hfx_ctx_destroy(hfx_ctx* c) {
   dfree(c->d_onee_items); dfree(c->d_sh_type); dfree(c->d_sh_np); dfree(c->d_sh_ao);
   dfree(c->d_sh_xyz); dfree(c->d_sh_z); dfree(c->d_sh_c); dfree(c->d_sh_cl);
   dfree(c->d_counts); dfree(c->d_stats); dfree(c->d_eblk); dfree(c->d_vmax); dfree(c->d_pmax); dfree(c->d_masks);
-  dfree(c->d_masks_side);
+  dfree(c->d_masks_side); dfree(c->d_cbuf);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   for (int p = 0; p < 2; ++p)

@@ -220,12 +220,19 @@ class Gen:
         # of a batch context; 0 with one molecule) and its threshold may be its own
         W("    const long long mq = pb.mbase; (void)mq;")
 
-    def emit_group(self, W, combos, ops_prim, ops_bra, ops_ctr, gid, sink=None):
+    def emit_group(self, W, combos, ops_prim, ops_bra, ops_ctr, gid, sink=None, fsel=None, hrr="ket_first",
+                   ext=None):
         """Primitive loops + contraction + HRR + spherical transform for the
         part combinations `combos`; returns {(ia,ib,ic,id): element var}.
         With `sink` (block kernels) the HRR runs ket first, then the bra per
         ket component, and sink(ia, ib, ic, id, var) is called as soon as an
-        element exists, so that it can be stored and dropped at once."""
+        element exists, so that it can be stored and dropped at once.
+        Split combinations (one part combination whose accumulators would not
+        fit the registers): fsel restricts the contracted targets [e0|f0] to
+        the ket components f in fsel; hrr="acc" then returns
+        {(e, f): accumulator} without an HRR, and hrr="cd_loop" runs the HRR
+        one ket output at a time, taking [e0|f0] for f outside fsel from
+        ext(e, f) (an expression, e.g. a load of what another kernel stored)."""
         wbra, wket = self.weights()
         bps = sorted({c[0] for c in combos})
         kps_by_bp = {bp: [c[1] for c in combos if c[0] == bp] for bp in bps}
@@ -239,15 +246,20 @@ class Gen:
         M = max(max(e_by_bp[bp]) for bp in bps) + max(max(f_by_kp[kp]) for kp in kps)
         targets = []
         seen = set()
+        def fl(kp):   # the ket components f of part kp that this kernel contracts
+            return [f for lf in f_by_kp[kp] for f in cart(lf) if fsel is None or f in fsel]
+
         for kp in kps:
             es = e_all if two_level else e_by_bp[bps[0]]
             for le in es:
-                for lf in f_by_kp[kp]:
-                    for e in cart(le):
-                        for f in cart(lf):
-                            if (e, f) not in seen:
-                                seen.add((e, f))
-                                targets.append((e, f))
+                for e in cart(le):
+                    for f in fl(kp):
+                        if (e, f) not in seen:
+                            seen.add((e, f))
+                            targets.append((e, f))
+        if fsel is not None:
+            assert len(combos) == 1
+            M = max(sum(e) + sum(f) for e, f in targets)
 
         def acc_name(bp, kp, e, f):
             return "acc%s_%d%d_%d%d_%s_%s" % (gid, bp[0], bp[1], kp[0], kp[1], tname(e), tname(f))
@@ -257,10 +269,9 @@ class Gen:
 
         for (bp, kp) in combos:
             for le in e_by_bp[bp]:
-                for lf in f_by_kp[kp]:
-                    for e in cart(le):
-                        for f in cart(lf):
-                            W("    R %s = 0;" % acc_name(bp, kp, e, f))
+                for e in cart(le):
+                    for f in fl(kp):
+                        W("    R %s = 0;" % acc_name(bp, kp, e, f))
         wa_f = "wb" if self.swap_bra else "wa"
         wb_f = "wa" if self.swap_bra else "wb"
         wc_f = "wb" if self.swap_ket else "wa"
@@ -333,15 +344,14 @@ class Gen:
                     continue
                 wk = "*".join(x for x in (wbra(bp), wk) if x)
                 for le in e_by_bp[bp]:
-                    for lf in f_by_kp[kp]:
-                        for e in cart(le):
-                            for f in cart(lf):
-                                v = vmap[(e, f)]
-                                if wk:
-                                    st = "%s += %s * %s;" % (acc_name(bp, kp, e, f), wk, v); ops_prim.n += 2
-                                else:
-                                    st = "%s += %s;" % (acc_name(bp, kp, e, f), v); ops_prim.n += 1
-                                accum.setdefault(v, []).append(st)
+                    for e in cart(le):
+                        for f in fl(kp):
+                            v = vmap[(e, f)]
+                            if wk:
+                                st = "%s += %s * %s;" % (acc_name(bp, kp, e, f), wk, v); ops_prim.n += 2
+                            else:
+                                st = "%s += %s;" % (acc_name(bp, kp, e, f), v); ops_prim.n += 1
+                            accum.setdefault(v, []).append(st)
         defined = {ln.split()[2] for ln in vlines}
         for v in list(accum):
             if v not in defined:           # a Boys term s<m> itself
@@ -366,6 +376,14 @@ class Gen:
                                 else:
                                     W("      %s += %s;" % (acc_name(bp, kp, e, f), t)); ops_bra.n += 1
         W("    }")
+        if hrr == "acc":
+            (bp, kp), = combos
+            return {(e, f): acc_name(bp, kp, e, f) for le in e_by_bp[bp] for e in cart(le) for f in fl(kp)}
+        if hrr == "cd_loop":
+            (bp, kp), = combos
+            have = {(e, f): acc_name(bp, kp, e, f) for le in e_by_bp[bp] for e in cart(le) for f in fl(kp)}
+            return self.emit_hrr_cd_loop(W, bp, kp, lambda e, f: have[(e, f)] if (e, f) in have else ext(e, f),
+                                         ops_ctr, gid, sink)
         # ---- HRR + spherical per part combination ----
         elem = {}
         cA, cB, cC, cD = comps(self.A), comps(self.B), comps(self.C), comps(self.D)
@@ -544,6 +562,106 @@ class Gen:
                     elem[(ia, ib, ic, id_)] = nm
                     sink(ia, ib, ic, id_, nm)
 
+    def emit_hrr_cd_loop(self, W, bp, kp, val, ops_ctr, gid, sink):
+        """HRR one ket output (ic, id) at a time, so that besides the
+        contracted [e0|f0] only the current output's (e|cd) values are live:
+          (e0|c,d+1_i) = (e0|c+1_i,d) + CD_i (e0|c,d)   for every bra index e,
+          (a,b+1_i|cd) = (a+1_i,b|cd) + AB_i (a,b|cd),
+        with the spherical transforms; val(e, f) is the expression of [e0|f0]
+        (an accumulator, or a load of what another kernel stored).  Ket
+        intermediates are recomputed per ket output (none are shared between
+        outputs for a p or s D shell)."""
+        cA, cB, cC, cD = comps(self.A), comps(self.B), comps(self.C), comps(self.D)
+        la, lb, er = self.erange(bp)
+        tag = "%s%d%d%d%d" % (gid, bp[0], bp[1], kp[0], kp[1])
+        el = [e for le in er for e in cart(le)]
+        kets = [(ic, cc, id_, cd) for ic, cc in enumerate(cC) if cc[0] == kp[0]
+                for id_, cd in enumerate(cD) if cd[0] == kp[1]]
+        elem = {}
+        for (ic, cc, id_, cd) in kets:
+            W("    {")
+            tc_ = [(cc[2], "1")] if cc[1] == "c" else SPH[cc[2]]
+            td_ = [(cd[2], "1")] if cd[1] == "c" else SPH[cd[2]]
+            base = {}    # [e0|f0] loaded or named once per ket output
+
+            def b(e_, f_):
+                key = (e_, f_)
+                if key not in base:
+                    nm = "cb%s_%s_%s" % (tag, tname(e_), tname(f_))
+                    W("      const R %s = %s;" % (nm, val(e_, f_)))
+                    base[key] = nm
+                return base[key]
+
+            ketname = {}
+            for e_ in el:
+                km = {}
+
+                def hk(c_, d_, e_=e_, km=km):
+                    key = (c_, d_)
+                    if key in km:
+                        return km[key]
+                    if sum(d_) == 0:
+                        km[key] = b(e_, c_)
+                        return km[key]
+                    i = next(k for k in range(3) if d_[k] > 0)
+                    x1 = hk(add1(c_, i), sub1(d_, i))
+                    x2 = hk(c_, sub1(d_, i))
+                    nm = "hk%s_%s_%s_%s" % (tag, tname(e_), tname(c_), tname(d_))
+                    W("      const R %s = %s + CD%s * %s;" % (nm, x1, AX[i], x2))
+                    ops_ctr.n += 2
+                    km[key] = nm
+                    return nm
+
+                terms = [(hk(x, y), fx, fy) for (x, fx) in tc_ for (y, fy) in td_]
+                if len(terms) == 1 and terms[0][1] == "1" and terms[0][2] == "1":
+                    ketname[e_] = terms[0][0]
+                else:
+                    nm = "ks%s_%s" % (tag, tname(e_))
+                    ex = " + ".join("(R)(%s*%s) * %s" % (fx, fy, t) if (fx, fy) != ("1", "1") else t
+                                    for (t, fx, fy) in terms)
+                    W("      const R %s = %s;" % (nm, ex))
+                    ops_ctr.n += 2 * len(terms) - 1
+                    ketname[e_] = nm
+            hm = {}
+
+            def hb(a_, b_, hm=hm, ketname=ketname):
+                key = (a_, b_)
+                if key in hm:
+                    return hm[key]
+                if sum(b_) == 0:
+                    hm[key] = ketname[a_]
+                    return hm[key]
+                i = next(k for k in range(3) if b_[k] > 0)
+                x1 = hb(add1(a_, i), sub1(b_, i))
+                x2 = hb(a_, sub1(b_, i))
+                nm = "hb%s_%s_%s" % (tag, tname(a_), tname(b_))
+                W("      const R %s = %s + AB%s * %s;" % (nm, x1, AX[i], x2))
+                ops_ctr.n += 2
+                hm[key] = nm
+                return nm
+
+            for ia, ca in enumerate(cA):
+                if ca[0] != bp[0]:
+                    continue
+                for ib, cb in enumerate(cB):
+                    if cb[0] != bp[1]:
+                        continue
+                    ta = [(ca[2], "1")] if ca[1] == "c" else SPH[ca[2]]
+                    tb = [(cb[2], "1")] if cb[1] == "c" else SPH[cb[2]]
+                    terms = [(hb(x, y), fx, fy) for (x, fx) in ta for (y, fy) in tb]
+                    nm = "e_%d_%d_%d_%d" % (ia, ib, ic, id_)
+                    if len(terms) == 1 and terms[0][1] == "1" and terms[0][2] == "1":
+                        W("      const R %s = %s;" % (nm, terms[0][0]))
+                    else:
+                        ex = " + ".join("(R)(%s*%s) * %s" % (fx, fy, t) if (fx, fy) != ("1", "1") else t
+                                        for (t, fx, fy) in terms)
+                        W("      const R %s = %s;" % (nm, ex))
+                        ops_ctr.n += 2 * len(terms) - 1
+                    elem[(ia, ib, ic, id_)] = nm
+                    sink(ia, ib, ic, id_, nm)
+            W("    }")
+        return elem
+
     def ref_index(self, ia, ib, ic, id_):
         n_ref = [NCOMP[x] for x in self.cls]
         i1, i2 = (ib, ia) if self.swap_bra else (ia, ib)
@@ -582,6 +700,23 @@ class Gen:
             if cur:
                 groups.append(cur)
         return groups
+
+    def split_parts(self, combos, fp32=False):
+        """A group that is one part combination whose contracted accumulators
+        exceed SPLIT_ACC (FP64) or 2 x SPLIT_ACC (FP32): its ket components
+        f in near-equal parts, each part's [e0|f0] held by one kernel (the
+        pre-kernels store theirs, the last kernel runs the HRR); else None."""
+        if len(combos) != 1 or not SPLIT_CLASSES.get(self.cls, (False, False))[1 if fp32 else 0]:
+            return None
+        (bp, kp), = combos
+        ne = sum((l + 1) * (l + 2) // 2 for l in self.erange(bp)[2])
+        fl = [f for lf in self.frange(kp)[2] for f in cart(lf)]
+        budget = SPLIT_ACC * (2 if fp32 else 1)
+        if ne * len(fl) <= budget:
+            return None
+        k = -(-ne * len(fl) // budget)          # parts
+        size = -(-len(fl) // k)
+        return [fl[i:i + size] for i in range(0, len(fl), size)]
 
     def group_M(self, combos):
         return max(self.combo_M(c) for c in combos)
@@ -766,10 +901,11 @@ class Gen:
         nA, nB, nC, nD = self.dims()
         n = 0
         kname = "f" if fp32 else "c"
-        for gi, combos in enumerate(plan if plan is not None else self.group_plan(True, fp32)):
-            M = self.group_M(combos)
+        goff = 0   # contracted values stored by the pre-kernels of the class, per quartet, so far
+
+        def head(name, M):
             W("template <typename R>")
-            W("__global__ void __launch_bounds__(HFX_NT) eri_%s_%s%d(const ClassArgs a) {" % (cls, kname, gi))
+            W("__global__ void __launch_bounds__(HFX_NT) %s(const ClassArgs a) {" % name)
             W("  extern __shared__ char hfx_smem[];")
             W("  if (a.q0 >= *a.count) return;   // a chunk past the list's end (chunks sized by capacity)")
             W("  R* btab = reinterpret_cast<R*>(hfx_smem);")
@@ -778,7 +914,43 @@ class Gen:
             W("  const int qend = min(a.q1, *a.count);")
             W("  for (int iq = a.q0 + blockIdx.x * blockDim.x + threadIdx.x; iq < qend; iq += gridDim.x * blockDim.x) {")
             W("    const int2 bk = a.list[iq];")
-            W("    R* out = reinterpret_cast<R*>(a.eblk) + (size_t)((iq - a.q0) >> 6) * %d + ((iq - a.q0) & 63);" % (nA * nB * nC * nD * 64))
+            W("    const size_t slot = (size_t)(iq - a.q0);")
+            W("    R* cb = reinterpret_cast<R*>(a.cbuf); (void)cb;   // contracted values of split combinations")
+
+        def tail(name):
+            W("  }")
+            W("}")
+            W("template __global__ void %s<%s>(const ClassArgs);" % (name, "float" if fp32 else "double"))
+
+        for gi, combos in enumerate(plan if plan is not None else self.group_plan(True, fp32)):
+            M = self.group_M(combos)
+            parts = self.split_parts(combos, fp32)
+            kern = "eri_%s_%s%d" % (cls, kname, gi)
+            ext = None
+            fsel = None
+            if parts is not None:
+                # pre-kernels: parts[:-1] of the ket components, contracted and stored
+                (bp, kp), = combos
+                el = [e for le in self.erange(bp)[2] for e in cart(le)]
+                stored = [f for part in parts[:-1] for f in part]
+                idx = {}
+                for e in el:
+                    for f in stored:
+                        idx[(e, f)] = goff + len(idx)
+                for pi, part in enumerate(parts[:-1]):
+                    pname = "%sv%d" % (kern, pi)
+                    head(pname, M)
+                    self.emit_quartet_setup(W)
+                    acc = self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%dv%d" % (gi, pi),
+                                          fsel=set(part), hrr="acc")
+                    for (e, f), nm in acc.items():
+                        W("    cb[(size_t)%d * a.vstride + slot] = %s;" % (idx[(e, f)], nm))
+                    tail(pname)
+                goff += len(idx)
+                fsel = set(parts[-1])
+                ext = lambda e, f, idx=idx: "cb[(size_t)%d * a.vstride + slot]" % idx[(e, f)]
+            head(kern, M)
+            W("    R* out = reinterpret_cast<R*>(a.eblk) + (slot >> 6) * %d + (slot & 63);" % (nA * nB * nC * nD * 64))
             self.emit_quartet_setup(W)
             # stored value: the diagonal builds drop |e| <= 1e-10 (fock_build_diag.f90:22);
             # the partial max|e| of the raw values feeds the block test
@@ -789,15 +961,18 @@ class Gen:
                 W("    { double o = (double)%s; vm = fmax(vm, fabs(o)); if (ecut && fabs(o) <= HFX_INTTHRESH) o = 0;"
                   " out[%d] = (R)o; }" % (nm, (((ia * nB + ib) * nC + ic) * nD + id_) * 64))
 
-            el = self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%d" % gi, sink=store)
+            if parts is None:
+                el = self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%d" % gi, sink=store)
+            else:
+                el = self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%d" % gi, sink=store, fsel=fsel,
+                                     hrr="cd_loop", ext=ext)
             if group_ops is not None:
                 group_ops.append((ops_prim.n, ops_bra.n, ops_ctr.n))
-            W("    a.vmax[(size_t)%d * a.vstride + (iq - a.q0)] = vm;" % gi)
+            W("    a.vmax[(size_t)%d * a.vstride + slot] = vm;" % gi)
             n += len(el)
-            W("  }")
-            W("}")
-            W("template __global__ void eri_%s_%s%d<%s>(const ClassArgs);" % (cls, kname, gi, "float" if fp32 else "double"))
+            tail(kern)
         assert n == nA * nB * nC * nD
+        return goff
 
     def kernel_digest(self, W, ops_ctr):
         """Digestion of the stored blocks of one block class: one thread per
@@ -1376,8 +1551,8 @@ class Gen:
             # evaluation and the shared recursion steps once per kernel
             union = [[(bp, kp) for bp in self.bra_parts for kp in self.ket_parts]]
             self.kernel_block(lambda _line: None, ops_prim, ops_bra, ops_ctr, plan=union)
-            self.kernel_block(W, ex_prim, ex_bra, ex_ctr)
-            self.kernel_block(W, Ops(), Ops(), Ops(), fp32=True)
+            cb64 = self.kernel_block(W, ex_prim, ex_bra, ex_ctr)
+            cb32 = self.kernel_block(W, Ops(), Ops(), Ops(), fp32=True)
             nA, nB, nC, nD = self.dims()
             # digestion in dig_<class>: 6 contractions of every element + block max
             ops_ctr.n += 13 * nA * nB * nC * nD
@@ -1403,6 +1578,12 @@ class Gen:
             "exec_quartet": ex_ctr.n,
             "all_prims": self.all_prims,
         }
+        if split:
+            # pre-kernels per combination kernel (split combinations) and the
+            # contracted values they store per quartet
+            info["npre"] = [len(self.split_parts(g) or [0]) - 1 for g in self.group_plan(True)]
+            info["npre32"] = [len(self.split_parts(g, True) or [0]) - 1 for g in self.group_plan(True, True)]
+            info["cbuf64"], info["cbuf32"] = cb64, cb32
         return "\n".join(L) + "\n", info
 
     def digest_source(self):
@@ -1447,6 +1628,18 @@ class Gen:
 
 
 SPLIT_COST = 200    # classes whose fused kernel would keep more doubles live are split
+# Block classes (FP64 kernels, FP32 kernels) whose largest part combinations
+# are split over several kernels by ket component (Gen.split_parts) when
+# their contracted [e0|f0] exceed SPLIT_ACC doubles (2 x SPLIT_ACC floats):
+# pre-kernels store their parts, the last kernel runs the HRR one ket output
+# at a time.  Measured on the C60 build (class spans, ms, unsplit / split at
+# 128): (dd|ll) FP64 10.9 / 9.8, but (dl|dl) 22.2 / 24.1, (dd|dl) 18.8 / 20.8,
+# (dd|dd) 2.7 / 5.2, and the FP32 kernels of (dd|ll) 2.3 / 3.7 -- the last
+# kernel's loads of the stored parts and the repeated recursions cost more
+# than the spills they remove, once spilling kernels keep their scratch
+# memory (raise_scratch_limit in hfx_host.hip).
+SPLIT_ACC = 128
+SPLIT_CLASSES = {"ddll": (True, False)}
 
 # Digestion classes and precisions whose K contractions keep two passes over
 # the block (K1 per a, K2 per b) instead of one: measured on the C60 /
@@ -1606,6 +1799,10 @@ def main(outdir):
                 D.append("template <typename R> __global__ void eri_%s_c%d(const ClassArgs);" % (c, k))
             for k in range(table[c]["combos32"]):
                 D.append("template <typename R> __global__ void eri_%s_f%d(const ClassArgs);" % (c, k))
+            for kname, key in (("c", "npre"), ("f", "npre32")):
+                for k, m in enumerate(table[c][key]):
+                    for j in range(m):
+                        D.append("template <typename R> __global__ void eri_%s_%s%dv%d(const ClassArgs);" % (c, kname, k, j))
             D.append("template <bool JONLY, typename T, bool MULTI> __global__ void dig_%s(const DigestArgs);" % c)
             D.append("template <typename T> __global__ void dig1_%s(const DigestArgs);" % c)
         else:
@@ -1622,6 +1819,9 @@ def main(outdir):
     D.append("  const void* dig1; const void* dig1f;   // one-pass digestion (one molecule, J and K), FP64 / FP32 blocks")
     D.append("  int dig1_use;   // bit 0 / 1: the one-pass digestion measured faster for FP64 / FP32 blocks")
     D.append("  int dig1_min_run;   // ... when a list's mean run of quartets per bra pair is at least this")
+    D.append("  int npre[16]; const void* pre64[16][16];      // split combinations: kernels that run before c64[k]")
+    D.append("  int npre32[16]; const void* pre32[16][16];    // ... before c32[k]")
+    D.append("  int cbuf64, cbuf32;   // contracted values they store per quartet (doubles / floats)")
     D.append("};")
     D.append("static const ClassKernels kClassKernels[%d] = {" % len(names))
     for c in names:
@@ -1642,11 +1842,22 @@ def main(outdir):
                                     ", ".join("%.1ff" % x for x in t["combo_cost32"]))
             dg += ", (const void*)&dig1_%s<double>, (const void*)&dig1_%s<float>, %d, %d" % (
                 c, c, DIG1_USE.get(c, 0), DIG1_MIN_RUN.get(c, 1024))
+
+            def pre(npre, kname, tname_):
+                assert max(npre) <= 16, (c, npre)
+                cnt = "{" + ", ".join(str(x) for x in npre) + "}"
+                ptr = "{" + ", ".join("{" + ", ".join("(const void*)&eri_%s_%s%dv%d<%s>" % (c, kname, k, j, tname_)
+                                                      for j in range(m)) + "}" if m else "{nullptr}"
+                                      for k, m in enumerate(npre)) + "}"
+                return cnt + ", " + ptr
+            dg += ", %s, %s, %d, %d" % (pre(t["npre"], "c", "double"), pre(t["npre32"], "f", "float"),
+                                        t["cbuf64"], t["cbuf32"])
         else:
             fns = "(const void*)&eri_%s<double, false>, (const void*)&eri_%s<double, true>, (const void*)&eri_%s<float, false>" % (c, c, c)
             nc, cm, c64 = 0, "{0}", "{nullptr}"
             nc32, cm32, c32 = 0, "{0}", "{nullptr}"
-            dg = "nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, {0.0f}, {0.0f}, nullptr, nullptr, 0, 0"
+            dg = ("nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, {0.0f}, {0.0f}, nullptr, nullptr, 0, 0, "
+                  "{0}, {{nullptr}}, {0}, {{nullptr}}, 0, 0")
         D.append("  {%s, %d, %d, %d, %d, %d, %d, %d, {%d, %d, %d, %d}, %d, %d, %d, %s, %s, %d, %s, %s, %s}," % (
             fns, t["M"], int(t["all_prims"]), t["flops_prim"], t["flops_prim_fp32"], t["flops_braprim"],
             t["flops_quartet"], int(t["split"]), *t["dims"], int(t["swap_bra"]), int(t["swap_ket"]),

@@ -155,7 +155,20 @@ enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDevi
 struct hipDeviceProp_t {
   int multiProcessorCount;
   size_t sharedMemPerBlock;
+  int maxThreadsPerMultiProcessor;
 };
+// no scratch memory on the CPU: kernels report none and the limits are unsupported
+struct hipFuncAttributes {
+  size_t localSizeBytes;
+};
+enum hipLimit_t { hipLimitStackSize = 0, hipExtLimitScratchMin = 0x1000, hipExtLimitScratchMax = 0x1001,
+                  hipExtLimitScratchCurrent = 0x1002 };
+inline hipError_t hipFuncGetAttributes(hipFuncAttributes* a, const void*) {
+  a->localSizeBytes = 0;
+  return 0;
+}
+inline hipError_t hipDeviceGetLimit(size_t*, hipLimit_t) { return 1; }
+inline hipError_t hipDeviceSetLimit(hipLimit_t, size_t) { return 1; }
 inline const char* hipGetErrorString(hipError_t) { return "emulated"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipMalloc(void** p, size_t n) {
@@ -269,6 +282,7 @@ inline hipError_t hipGetDevice(int* d) {
 inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
   p->multiProcessorCount = 1;
   p->sharedMemPerBlock = HFX_EMU_SMEM_BYTES;
+  p->maxThreadsPerMultiProcessor = 64;
   return hipSuccess;
 }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
