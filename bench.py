@@ -80,7 +80,11 @@ def _class_kernels(cls):
         with open(path) as fh:
             split = json.load(fh)["classes"][cls]["split"]
     if split:
-        return "class (%s|%s) FP64 span: eri_%s_c*<double> + dig_%s<false,double>" % (cls[:2], cls[2:], cls, cls)
+        # the digestion form the library picks for a one-molecule FP64 list (table of
+        # measured classes, with long enough bra runs, which C60's lists have)
+        from hfcode_amd.gen.gen_eri import DIG1_USE
+        dig = ("dig1_%s<double>" % cls) if DIG1_USE.get(cls, 0) & 1 else ("dig_%s<false,double>" % cls)
+        return "class (%s|%s) FP64 span: eri_%s_c*<double> + %s" % (cls[:2], cls[2:], cls, dig)
     return "class (%s|%s) FP64: eri_%s<double,false>" % (cls[:2], cls[2:], cls)
 
 
