@@ -571,10 +571,11 @@ ClassArgs base_args(hfx_ctx* c, const double* dP, double thr) {
   return a;
 }
 
-int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long long cap) {
+int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long long cap, hipStream_t st = nullptr) {
   const ClassKernels& k = kClassKernels[cls];
+  if (!st) st = c->stream;
   const bool prof = c->profile && !c->device_chunks;
-  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
+  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][0], st));
   const void* fn = prec == 1 ? k.f32 : (jonly ? k.f64j : k.f64);
   const int K = prec == 1 ? 5 : 8;
   const size_t boys_shm = (size_t)HFX_BOYS_ROWS * (k.M + K) * (prec == 1 ? sizeof(float) : sizeof(double));
@@ -590,8 +591,8 @@ int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   if (c->cls_per_cu[cls] > 0) per_cu = c->cls_per_cu[cls];   // tuned launch width
   const int grid = grid_for(cap, HFX_NT, c->ncu * per_cu);
   void* args[] = {&a};
-  HIPOK(hipLaunchKernel(fn, dim3(grid), dim3(HFX_NT), args, shm, c->stream));
-  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));
+  HIPOK(hipLaunchKernel(fn, dim3(grid), dim3(HFX_NT), args, shm, st));
+  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][1], st));
   c->class_ran[prec][cls] = 1;
   return HFX_OK;
 }
@@ -757,7 +758,9 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
       nside = std::min(ncombo, (int)hfx_ctx::kSide);
       if (j >= 2)
         for (int s = 0; s < nside; ++s) HIPOK(hipStreamWaitEvent(c->side[s], c->done[h], 0));
-    } else if (ncombo > 1 && q1 - q0 >= kConcurrentChunk && live_contexts.load() == 1 && !c->device_chunks) {
+    } else if (ncombo > 1 && (q1 - q0 >= kConcurrentChunk || a.diag) && live_contexts.load() == 1 &&
+               !c->device_chunks) {
+      // (the diagonal builds' chunks are small but their kernels few and latency-bound)
       nside = std::min(ncombo - 1, (int)hfx_ctx::kSide);
       int r = ensure_side_streams(c);
       if (r) return r;
@@ -1056,10 +1059,23 @@ int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initia
   if (!c->capturing) HIPOK(hipEventRecord(c->ev0, c->stream));
   if ((r = zero_build_state(c))) return r;
   if ((r = launch_maxdm(c, dP))) return r;
+  // The pair types' screens and classes are small, latency-bound launches:
+  // with one live context and launches sized on the host, the fused classes
+  // (no integral buffer) run on the side streams beside the block classes on
+  // the context's stream, and join before X is folded into F.
+  const bool conc = live_contexts.load() == 1 && !c->device_chunks && !c->capturing;
+  int nside_used = 0;
+  if (conc) {
+    if ((r = ensure_side_streams(c))) return r;
+    HIPOK(hipEventRecord(c->fork, c->stream));
+    for (int k = 0; k < hfx_ctx::kSide; ++k) HIPOK(hipStreamWaitEvent(c->side[k], c->fork, 0));
+  }
   for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
     const int n = c->npair[t];
     if (n == 0) continue;
     const int cls = kDiagClass[t];
+    hipStream_t st = c->stream;
+    if (conc && !kClassKernels[cls].split && !c->debug_blocks) st = c->side[nside_used++ % hfx_ctx::kSide];
     DiagScreenArgs s;
     memset(&s, 0, sizeof(s));
     s.pairs = c->d_pairs;
@@ -1076,7 +1092,7 @@ int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initia
     s.nsh = c->nsp;
     s.check = initial ? 0 : 1;
     s.all_prims = kClassKernels[cls].all_prims;
-    hipLaunchKernelGGL(k_screen_diag, dim3(grid_for(n, 256, 1024)), dim3(256), 0, c->stream, s);
+    hipLaunchKernelGGL(k_screen_diag, dim3(grid_for(n, 256, 1024)), dim3(256), 0, st, s);
     HIPOK(hipGetLastError());
     ClassArgs a = base_args(c, dP, thr);
     a.thr_mol = thr_mol;
@@ -1096,8 +1112,14 @@ int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initia
     if (kClassKernels[cls].split)
       r = launch_split(c, cls, 0, false, a, n, a.blocks);
     else
-      r = launch_class(c, cls, 0, false, a, n);
+      r = launch_class(c, cls, 0, false, a, n, st);
     if (r) return r;
+  }
+  if (conc) {
+    for (int k = 0; k < hfx_ctx::kSide; ++k) {
+      HIPOK(hipEventRecord(c->join[k], c->side[k]));
+      HIPOK(hipStreamWaitEvent(c->stream, c->join[k], 0));
+    }
   }
   hipLaunchKernelGGL(k_pairbuf_to_x, dim3(grid_for((long long)c->totpairs * HFX_PAIRBUF_STRIDE, 256, 2048)), dim3(256), 0,
                      c->stream, c->d_pairbuf, c->d_pairs, c->d_shells, c->totpairs, c->d_X, c->nb);
