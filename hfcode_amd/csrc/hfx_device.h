@@ -107,12 +107,41 @@ __device__ inline int pick_resident(const int2* list, int lo, int hi, int cur) {
 #ifndef HFX_PIN
 #define HFX_PIN(x) asm volatile("" : "+v"(x))
 #endif
+// The same for a pointer into global memory, keeping its address space: an
+// opaque generic pointer would turn the loads through it into flat loads,
+// which also count against the LDS wait counter, so that every LDS wait
+// (atomics, row reads) would first drain them.
+#ifndef HFX_PIN_GPTR
+template <typename T>
+__device__ __forceinline__ const T* hfx_pin_global(const T* p) {
+  using GT = const __attribute__((address_space(1))) T*;
+  GT g = (GT)p;
+  asm volatile("" : "+v"(g));
+  return (const T*)g;
+}
+#define HFX_PIN_GPTR(p) ((p) = hfx_pin_global(p))
+#endif
 
 // Wave-level barrier: LDS written by one lane is visible to the others.
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
+
+// 1/x and sqrt(1/x) of the primitive loop and the Boys function: the IEEE
+// forms in FP64; in FP32 the hardware's approximations (v_rcp_f32,
+// v_rsq_f32, 1 ulp), as the correctly rounded FP32 forms cost about as many
+// instructions as the FP64 ones and the FP32 path's results carry ~1e-7
+// relative error anyway.  hfx_sqrt_rcp(x, r) = sqrt(r) with r = 1/x.
+#ifndef HFX_RCPF
+#define HFX_RCPF(x) __builtin_amdgcn_rcpf(x)
+#define HFX_RSQF(x) __builtin_amdgcn_rsqf(x)
+#define HFX_EXP2F(x) __builtin_amdgcn_exp2f(x)
+#endif
+__device__ __forceinline__ double hfx_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float hfx_rcp(float x) { return HFX_RCPF(x); }
+__device__ __forceinline__ double hfx_sqrt_rcp(double, double r) { return sqrt(r); }
+__device__ __forceinline__ float hfx_sqrt_rcp(float x, float) { return HFX_RSQF(x); }
 
 template <typename R> struct BoysTerms;
 template <> struct BoysTerms<double> { static constexpr int K = 8; };
@@ -140,13 +169,13 @@ __device__ inline void stage_boys(R* tab, const ClassArgs& a, int ncol) {
 //  else:   Taylor series about the nearest grid point x_i,
 //          F_m(T) = sum_k F_{m+k}(x_i) (x_i - T)^k / k!, using dF_m/dx = -F_{m+1};
 //          K terms give < 1e-16 (double) / 1e-10 (float) relative error for
-//          |x_i - T| <= 1/32.
+//          |x_i - T| <= 1/32 (FP32: F_M only, then downward recursion).
 template <typename R, int M>
 __device__ __forceinline__ void boys(const R* __restrict__ tab, int ncol, R T, R* F) {
   constexpr int K = BoysTerms<R>::K;
   if (T > (R)HFX_BOYS_XMAX) {
-    const R t = (R)1 / T;
-    F[0] = (R)0.88622692545275801365 * sqrt(t);
+    const R t = hfx_rcp(T);
+    F[0] = (R)0.88622692545275801365 * hfx_sqrt_rcp(T, t);
 #pragma unroll
     for (int m = 1; m <= M; ++m) F[m] = F[m - 1] * ((R)m - (R)0.5) * t;
   } else {
@@ -158,12 +187,26 @@ __device__ __forceinline__ void boys(const R* __restrict__ tab, int ncol, R T, R
 #pragma unroll
     for (int k = 1; k < K; ++k) c[k] = c[k - 1] * d * (R)(1.0 / k);
     const R* row = tab + i * ncol;
+    if constexpr (sizeof(R) == sizeof(float) && M > 0) {
+      // FP32: F_M by the series, the lower orders by the downward recursion
+      // F_m = (2T F_{m+1} + exp(-T)) / (2m + 1) (positive terms: no
+      // cancellation, relative error <= (M + 1) ulp)
+      R s = row[M + K - 1] * c[K - 1];
 #pragma unroll
-    for (int m = 0; m <= M; ++m) {
-      R s = row[m + K - 1] * c[K - 1];
+      for (int k = K - 2; k >= 0; --k) s = fma(row[M + k], c[k], s);
+      F[M] = s;
+      const R e = HFX_EXP2F(T * (R)-1.44269504088896340736);
+      const R t2 = T + T;
 #pragma unroll
-      for (int k = K - 2; k >= 0; --k) s = fma(row[m + k], c[k], s);
-      F[m] = s;
+      for (int m = M - 1; m >= 0; --m) F[m] = fma(t2, F[m + 1], e) * (R)(1.0 / (2 * m + 1));
+    } else {
+#pragma unroll
+      for (int m = 0; m <= M; ++m) {
+        R s = row[m + K - 1] * c[K - 1];
+#pragma unroll
+        for (int k = K - 2; k >= 0; --k) s = fma(row[m + k], c[k], s);
+        F[m] = s;
+      }
     }
   }
 }

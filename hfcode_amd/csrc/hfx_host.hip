@@ -634,6 +634,10 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
                  long long runs = 0) {
   const ClassKernels& k = kClassKernels[cls];
   if (n <= 0) return HFX_OK;
+  if (prec == 1 && a.diag) {   // the diagonal builds' element cut (k_cut_small) is on FP64 blocks
+    fprintf(stderr, "hfx: FP32 blocks in a diagonal build\n");
+    return HFX_E_STATE;
+  }
   const long long ne = (long long)k.dims[0] * k.dims[1] * k.dims[2] * k.dims[3];
   // The integral buffer grows to what this class needs for all its quartets
   // at once (small molecules take megabytes), up to kEblkDoubles, or
@@ -713,9 +717,10 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   const size_t one_prow_off = rows_shm;
   const size_t one_tb_off = (one_prow_off + (size_t)(k.dims[0] + k.dims[1]) * c->nb *
                              (prec == 1 ? sizeof(float) : sizeof(double)) + 15) / 16 * 16;
-  const size_t one_run_off = one_tb_off + (size_t)(DIG1_NT / HFX_WAVE) * std::min(HFX_WAVE, 16) * k.dims[2] *
+  const int one_nt = prec == 1 ? k.dig1_nt32 : DIG1_NT;   // threads per one-pass workgroup
+  const size_t one_run_off = one_tb_off + (size_t)(one_nt / HFX_WAVE) * std::min(HFX_WAVE, 16) * k.dims[2] *
                                               k.dims[3] * sizeof(double);
-  const size_t one_shm = one_run_off + (size_t)(DIG1_NT / HFX_WAVE) * sizeof(int);
+  const size_t one_shm = one_run_off + (size_t)(one_nt / HFX_WAVE) * sizeof(int);
   // where it measured faster (the class's table entry for the list's
   // precision) and the list's bra runs are long (a step never crosses a bra
   // pair, so short runs leave most of the workgroup idle); runs = 0: unknown
@@ -795,6 +800,10 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
       HIPOK(hipEventRecord(c->join[s], c->side[s]));
       HIPOK(hipStreamWaitEvent(c->stream, c->join[s], 0));
     }
+    if (a.diag) {   // the diagonal builds' element cut, on the chunk's whole tiles
+      const long long nel = (q1 - q0 + 63) / 64 * 64 * ne;
+      hipLaunchKernelGGL(k_cut_small, dim3(grid_for(nel, 256, 2048)), dim3(256), 0, c->stream, eblk, nel);
+    }
     DigestArgs g;
     memset(&g, 0, sizeof(g));
     g.list = a.list;
@@ -836,13 +845,13 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
       int& occ1 = c->dig1_occ[prec == 1 ? 1 : 0][cls];
       if (occ1 == 0) {
         int nblk = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, fn1, DIG1_NT, one_shm) != hipSuccess || nblk < 1) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nblk, fn1, one_nt, one_shm) != hipSuccess || nblk < 1) {
           (void)hipGetLastError();
           nblk = 1;
         }
         occ1 = std::min(nblk, 4);
       }
-      HIPOK(hipLaunchKernel(fn1, dim3(c->ncu * occ1), dim3(DIG1_NT), gargs, one_shm, c->stream));
+      HIPOK(hipLaunchKernel(fn1, dim3(c->ncu * occ1), dim3(one_nt), gargs, one_shm, c->stream));
       if (pipelined) HIPOK(hipEventRecord(c->done[h], c->stream));
       continue;
     }

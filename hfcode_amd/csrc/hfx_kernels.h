@@ -108,6 +108,7 @@ __global__ void k_maxdm(const double* P, int nb, const ShellRec* sh, const int2*
 __global__ void k_screen(const ScreenArgs s);
 __global__ void k_screen_diag(const DiagScreenArgs s);
 __global__ void k_add_sym(const double* X, double* F, int nb, int nmat);
+__global__ void k_cut_small(double* e, long long n);
 __global__ void k_to_f32(const double* x, float* y, size_t n);
 // X += the pair buffer, block by block (every global pair once)
 __global__ void k_pairbuf_to_x(const double* pairbuf, const PairRec* pairs, const ShellRec* shells, int npairs,

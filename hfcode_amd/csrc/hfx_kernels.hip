@@ -296,6 +296,16 @@ __global__ void k_to_f32(const double* __restrict__ x, float* __restrict__ y, si
     y[i] = (float)x[i];
 }
 
+// Diagonal builds: integral elements with |e| <= 1e-10 are dropped
+// (fock_build_diag.f90:22) between a chunk's combination kernels and its
+// digestion; the block maxima were taken from the raw values.
+__global__ void k_cut_small(double* __restrict__ e, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double v = e[i];
+    if (v != 0.0 && fabs(v) <= HFX_INTTHRESH) e[i] = 0.0;
+  }
+}
+
 // F += X + X^T per nb x nb matrix (the accumulator holds each symmetric
 // contribution once); nmat stacked matrices.
 __global__ void k_add_sym(const double* __restrict__ X, double* __restrict__ F, int nb, int nmat) {

@@ -301,7 +301,7 @@ class Gen:
         W("        const R PQx = (R)(Pxd - rk.Px), PQy = (R)(Pyd - rk.Py), PQz = (R)(Pzd - rk.Pz);")
         if self.has_L_ket:
             W("        const R wC = (R)rk.%s, wD = (R)rk.%s; (void)wC; (void)wD;" % (wc_f, wd_f))
-        W("        const R ipq = (R)1 / (p + q);")
+        W("        const R ipq = hfx_rcp(p + q);")
         W("        const R rho_p = q * ipq, rho_q = p * ipq; (void)rho_p; (void)rho_q;")
         W("        const R T = p * rho_p * (PQx * PQx + PQy * PQy + PQz * PQz);")
         W("        const R oo2q = (R)0.5 * qinv, oo2pq = (R)0.5 * ipq; (void)oo2q; (void)oo2pq;")
@@ -312,7 +312,7 @@ class Gen:
         ops_prim.n += 1 + 1 + 2 + 6 + 2 + 6 + 6
         W("        R F[%d];" % (M + 1))
         W("        boys<R, %d>(btab, NB_COL, T, F);" % M)
-        W("        const R pref = Kab * Kcd * sqrt(ipq);")
+        W("        const R pref = Kab * Kcd * hfx_sqrt_rcp(p + q, ipq);")
         ops_prim.n += 3
         for m in range(M + 1):
             W("        const R s%d = pref * F[%d];" % (m, m))
@@ -952,14 +952,14 @@ class Gen:
             head(kern, M)
             W("    R* out = reinterpret_cast<R*>(a.eblk) + (slot >> 6) * %d + (slot & 63);" % (nA * nB * nC * nD * 64))
             self.emit_quartet_setup(W)
-            # stored value: the diagonal builds drop |e| <= 1e-10 (fock_build_diag.f90:22);
-            # the partial max|e| of the raw values feeds the block test
-            W("    const bool ecut = a.diag != 0;")
-            W("    double vm = 0;")
+            # raw values and their partial max|e| (the block test); the
+            # diagonal builds' element cut runs on the stored blocks
+            # (k_cut_small)
+            W("    R vm = 0;   // in the block's precision: FP32 maxima convert exactly")
 
             def store(ia, ib, ic, id_, nm):
-                W("    { double o = (double)%s; vm = fmax(vm, fabs(o)); if (ecut && fabs(o) <= HFX_INTTHRESH) o = 0;"
-                  " out[%d] = (R)o; }" % (nm, (((ia * nB + ib) * nC + ic) * nD + id_) * 64))
+                W("    { const R o = %s; vm = fmax(vm, fabs(o)); out[%d] = o; }" % (
+                    nm, (((ia * nB + ib) * nC + ic) * nD + id_) * 64))
 
             if parts is None:
                 el = self.emit_group(W, combos, ops_prim, ops_bra, ops_ctr, "g%d" % gi, sink=store)
@@ -968,7 +968,7 @@ class Gen:
                                      hrr="cd_loop", ext=ext)
             if group_ops is not None:
                 group_ops.append((ops_prim.n, ops_bra.n, ops_ctr.n))
-            W("    a.vmax[(size_t)%d * a.vstride + slot] = vm;" % gi)
+            W("    a.vmax[(size_t)%d * a.vstride + slot] = (double)vm;" % gi)
             n += len(el)
             tail(kern)
         assert n == nA * nB * nC * nD
@@ -1142,7 +1142,7 @@ class Gen:
         W("    {")
         pload("Pcd", "aoC", nC, "aoD", nD)
         zero("Jcd", ncd)
-        W("      const T* bp = blk; HFX_PIN(bp);")
+        W("      const T* bp = blk; HFX_PIN_GPTR(bp);")
         for ia in range(nA):
             W("      {")
             for ib in range(nB):
@@ -1199,7 +1199,7 @@ class Gen:
         # (nB rows, kept in registers) after the last a.  Rows of P for b are
         # reloaded per (a, b) from cache rather than held in registers.
         W("    {")
-        W("      const T* bp = blk; HFX_PIN(bp);")
+        W("      const T* bp = blk; HFX_PIN_GPTR(bp);")
         for ib in range(nB):
             zero("Kbc_%d" % ib, nC)
             zero("Kbd_%d" % ib, nD)
@@ -1235,7 +1235,7 @@ class Gen:
         W("    } else {")
         # pass K1
         W("    {")
-        W("      const T* bp = blk; HFX_PIN(bp);")
+        W("      const T* bp = blk; HFX_PIN_GPTR(bp);")
         pload("Pbd", "aoB", nB, "aoD", nD)
         pload("Pbc", "aoB", nB, "aoC", nC)
         for ia in range(nA):
@@ -1255,7 +1255,7 @@ class Gen:
         W("    }")
         # pass K2
         W("    {")
-        W("      const T* bp = blk; HFX_PIN(bp);")
+        W("      const T* bp = blk; HFX_PIN_GPTR(bp);")
         pload("Pad", "aoA", nA, "aoD", nD)
         pload("Pac", "aoA", nA, "aoC", nC)
         for ib in range(nB):
@@ -1318,8 +1318,11 @@ class Gen:
         orow = "o" if a_outer else "%d + o" % nA
         irow = (lambda i: "%d" % (nA + i)) if a_outer else (lambda i: "%d" % i)
 
+        nt32 = DIG1_NT32.get(cls, 512)
+        assert nt32 % 64 == 0 and 64 <= nt32 <= 1024
         W("template <typename T>")
-        W("__global__ void __launch_bounds__(DIG1_NT) dig1_%s(const DigestArgs d) {" % cls)
+        W("__global__ void __launch_bounds__(sizeof(T) == sizeof(double) ? DIG1_NT : %d) dig1_%s(const DigestArgs d) {"
+          % (nt32, cls))
         W("  extern __shared__ char hfx_smem[];")
         W("  constexpr int NR = %d;   // rows of A then rows of B" % (nA + nB))
         W("  double* srow = reinterpret_cast<double*>(hfx_smem);                 // [NR][nb] of X")
@@ -1412,7 +1415,7 @@ class Gen:
                       + " ".join("T Kid_%d_%d = 0;" % (i, dd) for dd in range(nD)))
             W("      #pragma unroll 1")
             W("      for (int o = 0; o < %d; ++o) {" % n_o)
-            W("        const T* bp = blk + (size_t)o * %d; HFX_PIN(bp);" % o_stride)
+            W("        const T* bp = blk + (size_t)o * %d; HFX_PIN_GPTR(bp);" % o_stride)
             W("        const T* po = prow + (size_t)(%s) * nbf;   // P row of the outer index" % orow)
             for dd in range(nD):
                 W("        const T Pod_%d = po[aoD + %d];" % (dd, dd))
@@ -1491,7 +1494,7 @@ class Gen:
         W("    if constexpr (sizeof(T) == sizeof(double)) {")
         core(True, cls in DIG1_PREFETCH64)
         W("    } else {")
-        core(False, DIG1_PREFETCH32)
+        core(cls in DIG1_KSLICE32, DIG1_PREFETCH32)
         W("    }")
         # X_cd through the wave's LDS staging into the ket pair's slot of the pair buffer
         W("    {")
@@ -1671,6 +1674,10 @@ DIG1_MIN_RUN = {"dddl": 512}
 # and (dd|dl) would spill 150 bytes per lane)
 DIG1_PREFETCH32 = True
 DIG1_PREFETCH64 = {"dlll", "llll"}
+# FP32 one-pass digestion: workgroup size per class (default 512 threads, as
+# the FP64 form), and the classes whose FP32 form flushes X_ic, X_id per slice
+DIG1_NT32 = {}
+DIG1_KSLICE32 = set()
 FP64_GROUP_ACC = 64    # block kernels: contracted accumulators per kernel, at most (FP64)
 FP32_GROUP_ACC = 128   # the same for the FP32 block kernels
 # per-class budgets picked by measurement (hfcode_amd/gen/autotune_groups.py);
@@ -1822,6 +1829,7 @@ def main(outdir):
     D.append("  int npre[16]; const void* pre64[16][16];      // split combinations: kernels that run before c64[k]")
     D.append("  int npre32[16]; const void* pre32[16][16];    // ... before c32[k]")
     D.append("  int cbuf64, cbuf32;   // contracted values they store per quartet (doubles / floats)")
+    D.append("  int dig1_nt32;   // threads per workgroup of the FP32 one-pass digestion (FP64: DIG1_NT)")
     D.append("};")
     D.append("static const ClassKernels kClassKernels[%d] = {" % len(names))
     for c in names:
@@ -1850,14 +1858,14 @@ def main(outdir):
                                                       for j in range(m)) + "}" if m else "{nullptr}"
                                       for k, m in enumerate(npre)) + "}"
                 return cnt + ", " + ptr
-            dg += ", %s, %s, %d, %d" % (pre(t["npre"], "c", "double"), pre(t["npre32"], "f", "float"),
-                                        t["cbuf64"], t["cbuf32"])
+            dg += ", %s, %s, %d, %d, %d" % (pre(t["npre"], "c", "double"), pre(t["npre32"], "f", "float"),
+                                            t["cbuf64"], t["cbuf32"], DIG1_NT32.get(c, 512))
         else:
             fns = "(const void*)&eri_%s<double, false>, (const void*)&eri_%s<double, true>, (const void*)&eri_%s<float, false>" % (c, c, c)
             nc, cm, c64 = 0, "{0}", "{nullptr}"
             nc32, cm32, c32 = 0, "{0}", "{nullptr}"
             dg = ("nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, {0.0f}, {0.0f}, nullptr, nullptr, 0, 0, "
-                  "{0}, {{nullptr}}, {0}, {{nullptr}}, 0, 0")
+                  "{0}, {{nullptr}}, {0}, {{nullptr}}, 0, 0, 0")
         D.append("  {%s, %d, %d, %d, %d, %d, %d, %d, {%d, %d, %d, %d}, %d, %d, %d, %s, %s, %d, %s, %s, %s}," % (
             fns, t["M"], int(t["all_prims"]), t["flops_prim"], t["flops_prim_fp32"], t["flops_braprim"],
             t["flops_quartet"], int(t["split"]), *t["dims"], int(t["swap_bra"]), int(t["swap_ket"]),

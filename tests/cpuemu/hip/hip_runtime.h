@@ -35,6 +35,10 @@
 #define HFX_WAVE 1
 #endif
 #define HFX_PIN(x) asm volatile("" : "+m"(x))
+#define HFX_PIN_GPTR(x) HFX_PIN(x)
+#define HFX_RCPF(x) (1.0f / (x))
+#define HFX_RSQF(x) (1.0f / std::sqrt(x))
+#define HFX_EXP2F(x) std::exp2(x)
 
 #define __global__
 #define __device__
