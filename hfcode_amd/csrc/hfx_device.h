@@ -143,6 +143,37 @@ __device__ __forceinline__ float hfx_rcp(float x) { return HFX_RCPF(x); }
 __device__ __forceinline__ double hfx_sqrt_rcp(double, double r) { return sqrt(r); }
 __device__ __forceinline__ float hfx_sqrt_rcp(float x, float) { return HFX_RSQF(x); }
 
+// Sum of x over the lanes of a wave, the same value in every lane (every
+// lane of the wave active).  A scan by DPP row shifts 1, 2, 4, 8 (zero fill)
+// leaves each row's total of 16 lanes in its lane 15; the four row totals
+// are read with readlane.  Cross-lane moves in the VALU: no LDS traffic and
+// none of the LDS latency of a ds_bpermute (__shfl_xor) tree.
+template <int CTRL>
+__device__ __forceinline__ double hfx_dpp(double x) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double hfx_readlane(double x, int l) {
+  const long long b = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double wave_sum(double x) {
+#if HFX_WAVE == 1
+  return x;
+#else
+  static_assert(HFX_WAVE == 64, "four rows of 16 lanes");
+  x += hfx_dpp<0x111>(x);   // row_shr:1
+  x += hfx_dpp<0x112>(x);   // row_shr:2
+  x += hfx_dpp<0x114>(x);   // row_shr:4
+  x += hfx_dpp<0x118>(x);   // row_shr:8
+  return (hfx_readlane(x, 15) + hfx_readlane(x, 31)) + (hfx_readlane(x, 47) + hfx_readlane(x, 63));
+#endif
+}
+
 template <typename R> struct BoysTerms;
 template <> struct BoysTerms<double> { static constexpr int K = 8; };
 template <> struct BoysTerms<float> { static constexpr int K = 5; };

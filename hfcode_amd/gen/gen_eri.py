@@ -852,7 +852,7 @@ class Gen:
                 W("      const double jab = wj * Jab_%d_%d;" % (i, j))
                 W("      if (rows) {")
                 W("        double r = inl ? jab : 0.0;")
-                W("        for (int o = HFX_WAVE / 2; o > 0; o >>= 1) r += __shfl_xor(r, o);")
+                W("        r = wave_sum(r);")
                 W("        if (lane == 0 && r != 0.0) atomicAdd(&srow[%d * nbf + rB + %d], r);" % (i, j))
                 W("      }")
                 W("      if (act && !inl) atomicAdd(&Xq[(size_t)(aoA + %d) * nbf + aoB + %d], jab);" % (i, j))
@@ -1128,7 +1128,7 @@ class Gen:
             W("      const double jd = wj * (double)jab;")
             W("      if (rows) {")
             W("        double r = inl ? jd : 0.0;")
-            W("        for (int o = HFX_WAVE / 2; o > 0; o >>= 1) r += __shfl_xor(r, o);")
+            W("        r = wave_sum(r);")
             W("        if (lane == 0 && r != 0.0) atomicAdd(&srow[%d * nbf + rB + %d], r);" % (ia, ib))
             W("      }")
             W("      if (act && !inl) atomicAdd(&XQ[(size_t)(aoA + %d) * nbf + aoB + %d], jd);" % (ia, ib))
@@ -1457,7 +1457,7 @@ class Gen:
                               i, c, dd, i, c, i, dd, c, i, dd))
                 # X_ab of this (a, b): summed over the wave (every lane has the resident bra)
                 W("          double r = wj * (double)jab;")
-                W("          for (int s = HFX_WAVE / 2; s > 0; s >>= 1) r += __shfl_xor(r, s);")
+                W("          r = wave_sum(r);")
                 if a_outer:
                     W("          if (lane == 0 && r != 0.0) atomicAdd(&srow[(size_t)o * nbf + rB + %d], r);" % i)
                 else:

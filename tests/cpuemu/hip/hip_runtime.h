@@ -12,6 +12,7 @@
 // loads the gfx950 build libhfx.so and nothing else.
 #pragma once
 #include <cmath>
+#include <cstdio>
 #include <cstddef>
 #include <cstdint>
 #include <cstdlib>
@@ -86,6 +87,8 @@ inline unsigned long long __ballot(bool p) { return p ? 1ull : 0ull; }
 inline int __lane_id() { return 0; }
 template <typename T> inline T __shfl(T v, int) { return v; }
 template <typename T> inline T __shfl_xor(T v, int) { return v; }
+inline int __builtin_amdgcn_update_dpp(int old, int, int, int, int, bool) { return old; }
+inline int __builtin_amdgcn_readlane(int v, int) { return v; }
 template <typename T> inline T __shfl_up(T v, unsigned) { return v; }
 template <typename T> inline T __shfl_down(T v, unsigned) { return v; }
 template <typename T> inline T atomicMax(T* p, T v) {
@@ -120,6 +123,20 @@ inline T hfx_lane_shfl(T v, int src, int site) {
   std::memcpy(&r, &b, sizeof(T));
   return r;
 }
+// DPP row_shr:k (ctrl 0x110 + k): lane l reads lane l - k of its row of 16
+// lanes, lanes with no such source keep `old`; readlane: lane l's value
+inline int hfx_lane_dpp(int old, int src, int ctrl, int site) {
+  if (ctrl <= 0x110 || ctrl > 0x11F) {
+    std::fprintf(stderr, "emulated DPP control 0x%x not supported\n", ctrl);
+    std::abort();
+  }
+  const int k = ctrl - 0x110, l = __lane_id();
+  const bool ok = (l & 15) >= k;
+  const int v = hfx_lane_shfl(src, ok ? l - k : l, site);
+  return ok ? v : old;
+}
+#define __builtin_amdgcn_update_dpp(old, src, ctrl, rm, bm, bc) hfx_lane_dpp((old), (src), (ctrl), __LINE__)
+#define __builtin_amdgcn_readlane(v, l) hfx_lane_shfl((int)(v), (int)(l), __LINE__)
 #define __ballot(p) hfx_lane_collective(HFX_LC_BALLOT, __LINE__, (p) ? 1ull : 0ull, 0)
 #define __shfl(v, s) hfx_lane_shfl((v), (int)(s), __LINE__)
 #define __shfl_xor(v, m) hfx_lane_shfl((v), __lane_id() ^ (int)(m), __LINE__)
