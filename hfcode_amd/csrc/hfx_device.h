@@ -41,6 +41,13 @@ struct PairRec {
 struct PrimRec {
   double p, pinv, Px, Py, Pz, K, wa, wb;
 };
+// The same in FP32 for the FP32 kernels, with P relative to the pair's two
+// centres (P - A, P - B, rounded from FP64): differences of centres are
+// formed per quartet in FP64, so no FP32 difference of absolute positions
+// loses digits.
+struct PrimRec32 {
+  float p, pinv, K, wa, wb, PAx, PAy, PAz, PBx, PBy, PBz, pad;
+};
 
 // Per-shell data the density-maximum kernel needs.
 struct ShellRec {
@@ -55,6 +62,7 @@ struct ClassArgs {
   const int* count;        // number of entries in list
   const PairRec* pairs;
   const PrimRec* prims;
+  const PrimRec32* prims32;   // the same records for the FP32 kernels
   const double* P;         // density, nb x nb
   double* X;               // accumulator; F += X + X^T at the end
   const double* boys64;    // HFX_BOYS_ROWS x HFX_BOYS_COLS

@@ -359,6 +359,7 @@ struct hfx_ctx {
   long long cls_chunk[HFX_NCLASSES] = {};     // quartets per chunk per block class (0: default)
   std::vector<PairRec> pairs;
   std::vector<PrimRec> prims;
+  std::vector<PrimRec32> prims32;   // prims in FP32, P relative to the pair's centres
   std::vector<ClassPlan> classes;
   int totpairs = 0;
   double split = 0.0;
@@ -392,6 +393,7 @@ struct hfx_ctx {
   hipEvent_t done[2] = {};        // digestion of the last chunk in each half of the integral buffer
   PairRec* d_pairs = nullptr;
   PrimRec* d_prims = nullptr;
+  PrimRec32* d_prims32 = nullptr;
   ShellRec* d_shells = nullptr;
   double* d_pairbuf = nullptr;    // X_cd of the digestion per ket pair (HFX_PAIRBUF_STRIDE per pair)
   double* d_boys64 = nullptr;
@@ -561,6 +563,7 @@ ClassArgs base_args(hfx_ctx* c, const double* dP, double thr) {
   memset(&a, 0, sizeof(a));
   a.pairs = c->d_pairs;
   a.prims = c->d_prims;
+  a.prims32 = c->d_prims32;
   a.P = dP;
   a.X = c->d_X;
   a.boys64 = c->d_boys64;
@@ -1377,6 +1380,7 @@ int hfx_batch_create(const hfx_mol_desc* descs, int nmol, hfx_ctx** out) {
     const double dx = A.x[0] - B.x[0], dy = A.x[1] - B.x[1], dz = A.x[2] - B.x[2];
     const double rab2 = dx * dx + dy * dy + dz * dz;
     std::vector<PrimRec> keep, neg;
+    std::vector<PrimRec32> keep32, neg32;
     for (int za = 0; za < A.nprim; ++za) {
       for (int zb = 0; zb < B.nprim; ++zb) {
         PrimRec r;
@@ -1391,7 +1395,16 @@ int hfx_batch_create(const hfx_mol_desc* descs, int nmol, hfx_ctx** out) {
         r.K = rt2pi54 * std::exp(-q * rab2) * r.pinv * dd;
         r.wa = A.type == 12 ? A.djkl[za] : 1.0;
         r.wb = B.type == 12 ? B.djkl[zb] : 1.0;
-        if (std::fabs(r.K) * std::sqrt(r.pinv) < kNegab) neg.push_back(r); else keep.push_back(r);
+        const PrimRec32 r32{(float)r.p, (float)r.pinv, (float)r.K, (float)r.wa, (float)r.wb,
+                            (float)(r.Px - A.x[0]), (float)(r.Py - A.x[1]), (float)(r.Pz - A.x[2]),
+                            (float)(r.Px - B.x[0]), (float)(r.Py - B.x[1]), (float)(r.Pz - B.x[2]), 0.0f};
+        if (std::fabs(r.K) * std::sqrt(r.pinv) < kNegab) {
+          neg.push_back(r);
+          neg32.push_back(r32);
+        } else {
+          keep.push_back(r);
+          keep32.push_back(r32);
+        }
       }
     }
     pr.p0 = (int)c->prims.size();
@@ -1399,6 +1412,8 @@ int hfx_batch_create(const hfx_mol_desc* descs, int nmol, hfx_ctx** out) {
     pr.npall = (int)(keep.size() + neg.size());
     c->prims.insert(c->prims.end(), keep.begin(), keep.end());
     c->prims.insert(c->prims.end(), neg.begin(), neg.end());
+    c->prims32.insert(c->prims32.end(), keep32.begin(), keep32.end());
+    c->prims32.insert(c->prims32.end(), neg32.begin(), neg32.end());
   }
   // classes
   for (int cl = 0; cl < HFX_NCLASSES; ++cl) {
@@ -1432,6 +1447,7 @@ int hfx_batch_create(const hfx_mol_desc* descs, int nmol, hfx_ctx** out) {
   boys_table(b64, b32);
   const size_t nmat = nb2 * nmol;
   if ((r = dalloc(&c->d_pairs, c->pairs.size())) || (r = dalloc(&c->d_prims, c->prims.size())) ||
+      (r = dalloc(&c->d_prims32, c->prims32.size())) ||
       (r = dalloc(&c->d_shells, sr.size())) || (r = dalloc(&c->d_molsh, molsh.size())) ||
       (r = dalloc(&c->d_pairbuf, (size_t)c->totpairs * HFX_PAIRBUF_STRIDE)) || (r = dalloc(&c->d_boys64, b64.size())) ||
       (r = dalloc(&c->d_boys32, b32.size())) || (r = dalloc(&c->d_P, nmat)) || (r = dalloc(&c->d_F, nmat)) ||
@@ -1445,6 +1461,8 @@ int hfx_batch_create(const hfx_mol_desc* descs, int nmol, hfx_ctx** out) {
       return fail(r);
   }
   if (copy_sync(c->stream, c->d_prims, c->prims.data(), c->prims.size() * sizeof(PrimRec), hipMemcpyHostToDevice) ||
+      copy_sync(c->stream, c->d_prims32, c->prims32.data(), c->prims32.size() * sizeof(PrimRec32),
+                hipMemcpyHostToDevice) ||
       copy_sync(c->stream, c->d_shells, sr.data(), sr.size() * sizeof(ShellRec), hipMemcpyHostToDevice) ||
       copy_sync(c->stream, c->d_molsh, molsh.data(), molsh.size() * sizeof(int2), hipMemcpyHostToDevice) ||
       copy_sync(c->stream, c->d_boys64, b64.data(), b64.size() * sizeof(double), hipMemcpyHostToDevice) ||
@@ -1479,7 +1497,7 @@ void hfx_ctx_destroy(hfx_ctx* c) {
     dfree(c->d_diag_list[t]); dfree(c->d_bmol[t]); dfree(c->d_aseg[t]);
   }
   dfree(c->d_molsh);
-  dfree(c->d_pairs); dfree(c->d_prims); dfree(c->d_shells); dfree(c->d_pairbuf); dfree(c->d_boys64); dfree(c->d_boys32);
+  dfree(c->d_pairs); dfree(c->d_prims); dfree(c->d_prims32); dfree(c->d_shells); dfree(c->d_pairbuf); dfree(c->d_boys64); dfree(c->d_boys32);
   dfree(c->d_P); dfree(c->d_P32); dfree(c->d_F); dfree(c->d_X); dfree(c->d_maxdm); dfree(c->d_schw_out);
   dfree(c->d_onee_items); dfree(c->d_sh_type); dfree(c->d_sh_np); dfree(c->d_sh_ao);
   dfree(c->d_sh_xyz); dfree(c->d_sh_z); dfree(c->d_sh_c); dfree(c->d_sh_cl);

@@ -213,6 +213,8 @@ class Gen:
         W("    const R ABx = (R)(Ax - pb.%sx), ABy = (R)(Ay - pb.%sy), ABz = (R)(Az - pb.%sz);" % (ob, ob, ob))
         W("    const R CDx = (R)(Cx - pk.%sx), CDy = (R)(Cy - pk.%sy), CDz = (R)(Cz - pk.%sz);" % (od, od, od))
         W("    (void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz;")
+        # FP32 primitive loops: P - Q = (A - C) + (P - A) - (Q - C), the centres' difference from FP64
+        W("    const R ACx = (R)(Ax - Cx), ACy = (R)(Ay - Cy), ACz = (R)(Az - Cz); (void)ACx; (void)ACy; (void)ACz;")
         W("    const int shA = pb.sh%s, shB = pb.sh%s, shC = pk.sh%s, shD = pk.sh%s;" % (oa, ob, oc, od))
         W("    const int aoA = pb.ao%s, aoB = pb.ao%s, aoC = pk.ao%s, aoD = pk.ao%s;" % (oa, ob, oc, od))
         W("    (void)shA; (void)shB; (void)shC; (void)shD; (void)aoA; (void)aoB; (void)aoC; (void)aoD;")
@@ -278,13 +280,27 @@ class Gen:
         wd_f = "wa" if self.swap_ket else "wb"
         npb = "pb.npall" if self.all_prims else "pb.np"
         npk = "pk.npall" if self.all_prims else "pk.np"
+        # primitive records: FP64 with absolute centres, FP32 (prims32) with
+        # the centre relative to each shell of the pair
+        pa_b = "PB" if self.swap_bra else "PA"
+        pa_k = "PB" if self.swap_ket else "PA"
         W("    for (int ib = 0; ib < %s; ++ib) {" % npb)
-        W("      const PrimRec& rb = a.prims[pb.p0 + ib];")
-        W("      const R p = (R)rb.p, pinv = (R)rb.pinv, Kab = (R)rb.K;")
-        W("      const double Pxd = rb.Px, Pyd = rb.Py, Pzd = rb.Pz;")
-        W("      const R PAx = (R)(Pxd - Ax), PAy = (R)(Pyd - Ay), PAz = (R)(Pzd - Az);")
+        W("      R p, pinv, Kab, PAx, PAy, PAz, wA = 1, wB = 1; (void)wA; (void)wB;")
+        W("      double Pxd = 0, Pyd = 0, Pzd = 0; (void)Pxd; (void)Pyd; (void)Pzd;")
+        W("      if constexpr (sizeof(R) == sizeof(double)) {")
+        W("        const PrimRec& rb = a.prims[pb.p0 + ib];")
+        W("        p = rb.p; pinv = rb.pinv; Kab = rb.K;")
+        W("        Pxd = rb.Px; Pyd = rb.Py; Pzd = rb.Pz;")
+        W("        PAx = (R)(Pxd - Ax); PAy = (R)(Pyd - Ay); PAz = (R)(Pzd - Az);")
         if self.has_L_bra:
-            W("      const R wA = (R)rb.%s, wB = (R)rb.%s; (void)wA; (void)wB;" % (wa_f, wb_f))
+            W("        wA = (R)rb.%s; wB = (R)rb.%s;" % (wa_f, wb_f))
+        W("      } else {")
+        W("        const PrimRec32& rb = a.prims32[pb.p0 + ib];")
+        W("        p = rb.p; pinv = rb.pinv; Kab = rb.K;")
+        W("        PAx = rb.%sx; PAy = rb.%sy; PAz = rb.%sz;" % (pa_b, pa_b, pa_b))
+        if self.has_L_bra:
+            W("        wA = rb.%s; wB = rb.%s;" % (wa_f, wb_f))
+        W("      }")
         W("      const R oo2p = (R)0.5 * pinv; (void)oo2p;")
         ops_bra.n += 4
         if two_level:
@@ -295,12 +311,22 @@ class Gen:
                             for f in cart(lf):
                                 W("      R %s = 0;" % tmp_name(kp, e, f))
         W("      for (int ik = 0; ik < %s; ++ik) {" % npk)
-        W("        const PrimRec& rk = a.prims[pk.p0 + ik];")
-        W("        const R q = (R)rk.p, qinv = (R)rk.pinv, Kcd = (R)rk.K;")
-        W("        const R QCx = (R)(rk.Px - Cx), QCy = (R)(rk.Py - Cy), QCz = (R)(rk.Pz - Cz);")
-        W("        const R PQx = (R)(Pxd - rk.Px), PQy = (R)(Pyd - rk.Py), PQz = (R)(Pzd - rk.Pz);")
+        W("        R q, qinv, Kcd, QCx, QCy, QCz, PQx, PQy, PQz, wC = 1, wD = 1; (void)wC; (void)wD;")
+        W("        if constexpr (sizeof(R) == sizeof(double)) {")
+        W("          const PrimRec& rk = a.prims[pk.p0 + ik];")
+        W("          q = rk.p; qinv = rk.pinv; Kcd = rk.K;")
+        W("          QCx = (R)(rk.Px - Cx); QCy = (R)(rk.Py - Cy); QCz = (R)(rk.Pz - Cz);")
+        W("          PQx = (R)(Pxd - rk.Px); PQy = (R)(Pyd - rk.Py); PQz = (R)(Pzd - rk.Pz);")
         if self.has_L_ket:
-            W("        const R wC = (R)rk.%s, wD = (R)rk.%s; (void)wC; (void)wD;" % (wc_f, wd_f))
+            W("          wC = (R)rk.%s; wD = (R)rk.%s;" % (wc_f, wd_f))
+        W("        } else {")
+        W("          const PrimRec32& rk = a.prims32[pk.p0 + ik];")
+        W("          q = rk.p; qinv = rk.pinv; Kcd = rk.K;")
+        W("          QCx = rk.%sx; QCy = rk.%sy; QCz = rk.%sz;" % (pa_k, pa_k, pa_k))
+        W("          PQx = (ACx + PAx) - QCx; PQy = (ACy + PAy) - QCy; PQz = (ACz + PAz) - QCz;")
+        if self.has_L_ket:
+            W("          wC = rk.%s; wD = rk.%s;" % (wc_f, wd_f))
+        W("        }")
         W("        const R ipq = hfx_rcp(p + q);")
         W("        const R rho_p = q * ipq, rho_q = p * ipq; (void)rho_p; (void)rho_q;")
         W("        const R T = p * rho_p * (PQx * PQx + PQy * PQy + PQz * PQz);")
