@@ -119,6 +119,10 @@ __device__ inline unsigned long long wave_sum(unsigned long long a) {
 // which the digestion of the block classes relies on to accumulate rows of
 // X in LDS.
 __global__ void k_screen(const ScreenArgs s) {
+  // per wave (four per workgroup): survivors per ket key (FP64, FP32 list),
+  // then each key's next list position (FP64, FP32); lane 0 writes them
+  __shared__ int kcount[4][4][128];   // launched with 256 threads
+  int (*kw)[128] = kcount[(threadIdx.x / HFX_WAVE) & 3];
   const int lane = __lane_id();
   const int nw = gridDim.x * (blockDim.x / HFX_WAVE);
   const int ns = s.nsh;
@@ -156,11 +160,15 @@ __global__ void k_screen(const ScreenArgs s) {
     unsigned long long* mrow = s.masks + (size_t)i * s.nchunk * 2;
     int n64 = 0, n32 = 0;
     unsigned long long pq64 = 0, bp64 = 0, pq32 = 0, bp32 = 0;
-    unsigned long long klo = 0, khi = 0;   // set of survivors' primitive counts + 1 (<= 101)
-    #pragma unroll 4
+    // a survivor's key: its ket's primitive-pair count + 1 (1..101); the
+    // keys present (uniform), and each key's survivors counted in kw[0..1]
+    unsigned long long klo = 0, khi = 0;
+    for (int e = lane; e < 256; e += HFX_WAVE) kw[e >> 7][e & 127] = 0;
+    wave_sync();
     for (int j0 = kb; j0 < jcut; j0 += HFX_WAVE) {
       const int j = j0 + lane;
       bool k64 = false, k32 = false;
+      int key = 0;
       if (j < jcut) {
         const int4 k = s.kinfo[j];
         double mv = fmax(md[(size_t)b.x * ns + b.y], md[(size_t)k.x * ns + k.y]);
@@ -175,8 +183,7 @@ __global__ void k_screen(const ScreenArgs s) {
           const unsigned long long nb_ = s.all_prims ? b.w : b.z;
           const unsigned long long nk_ = s.all_prims ? k.w : k.z;
           if (k64) { pq64 += nb_ * nk_; bp64 += nb_; } else { pq32 += nb_ * nk_; bp32 += nb_; }
-          const int key = (int)nk_ + 1;
-          if (key < 64) klo |= 1ull << key; else khi |= 1ull << (key - 64);
+          key = (int)nk_ + 1;
         }
       }
       const unsigned long long m64 = __ballot(k64), m32 = __ballot(k32);
@@ -186,12 +193,20 @@ __global__ void k_screen(const ScreenArgs s) {
       }
       n64 += __popcll(m64);
       n32 += __popcll(m32);
+      // the chunk's survivors counted key by key (each pass takes every lane
+      // with the first remaining lane's key)
+      for (unsigned long long rem = m64 | m32; rem;) {
+        const int kk = __builtin_amdgcn_readlane(key, __ffsll((long long)rem) - 1);
+        const unsigned long long same = __ballot(key == kk) & rem;
+        if (lane == 0) {
+          kw[0][kk] += __popcll(same & m64);
+          kw[1][kk] += __popcll(same & m32);
+        }
+        if (kk < 64) klo |= 1ull << kk; else khi |= 1ull << (kk - 64);
+        rem &= ~same;
+      }
     }
     pq64 = wave_sum(pq64); bp64 = wave_sum(bp64); pq32 = wave_sum(pq32); bp32 = wave_sum(bp32);
-    for (int o = HFX_WAVE / 2; o > 0; o >>= 1) {
-      klo |= __shfl_xor(klo, o);
-      khi |= __shfl_xor(khi, o);
-    }
     int base64 = 0, base32 = 0;
     if (lane == 0) {
       if (n64) {
@@ -211,35 +226,55 @@ __global__ void k_screen(const ScreenArgs s) {
     }
     base64 = __shfl(base64, 0);
     base32 = __shfl(base32, 0);
-    wave_sync();   // the masks lane 0 stored are read by every lane
-    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     // Survivors grouped by ket primitive count, largest first (ket order
     // inside a group): the class kernels loop over a quartet's primitive
     // pairs, and a wave then runs quartets of equal trip count together.
-    // One scan of the bra's masks per distinct count (the set came from the
-    // mask pass).
-    while (klo | khi) {
-      const int key = khi ? 127 - __clzll((long long)khi) : 63 - __clzll((long long)klo);
-      if (key >= 64) khi &= ~(1ull << (key - 64)); else klo &= ~(1ull << key);
-      #pragma unroll 4
-      for (int j0 = kb; j0 < jcut; j0 += HFX_WAVE) {
-        const unsigned long long m64 = mrow[((j0 - kb) / HFX_WAVE) * 2];
-        const unsigned long long m32 = mrow[((j0 - kb) / HFX_WAVE) * 2 + 1];
-        bool sel = false;
-        if ((m64 | m32) >> lane & 1ull) {
-          const int4 k = s.kinfo[j0 + lane];
-          sel = (s.all_prims ? k.w : k.z) + 1 == key;
-        }
-        const unsigned long long s64 = __ballot(sel && (m64 >> lane & 1ull));
-        const unsigned long long s32 = __ballot(sel && !(m64 >> lane & 1ull));
-        if (sel) {
-          const int2 v = make_int2(gb, s.ket[j0 + lane]);
-          if (m64 >> lane & 1ull) s.list64[base64 + __popcll(s64 & below)] = v;
-          else s.list32[base32 + __popcll(s32 & below)] = v;
-        }
-        base64 += __popcll(s64);
-        base32 += __popcll(s32);
+    // Each present key's first list position, keys in descending order ...
+    if (lane == 0) {
+      for (unsigned long long hl = khi, ll = klo; hl | ll;) {
+        int kk;
+        if (hl) { kk = 127 - __clzll((long long)hl); hl &= ~(1ull << (kk - 64)); }
+        else { kk = 63 - __clzll((long long)ll); ll &= ~(1ull << kk); }
+        kw[2][kk] = base64;
+        kw[3][kk] = base32;
+        base64 += kw[0][kk];
+        base32 += kw[1][kk];
       }
+    }
+    wave_sync();   // positions and the masks lane 0 stored are read by every lane
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // ... then one scan of the bra's masks, each chunk's survivors placed key
+    // by key (a chunk's keys are distinct passes, so its position reads
+    // precede lane 0's updates of the same key)
+    for (int j0 = kb; j0 < jcut; j0 += HFX_WAVE) {
+      const unsigned long long m64 = mrow[((j0 - kb) / HFX_WAVE) * 2];
+      const unsigned long long m32 = mrow[((j0 - kb) / HFX_WAVE) * 2 + 1];
+      if ((m64 | m32) == 0ull) continue;
+      const bool surv = (m64 | m32) >> lane & 1ull;
+      int key = 0;
+      int2 v = make_int2(0, 0);
+      if (surv) {
+        const int4 k = s.kinfo[j0 + lane];
+        key = (s.all_prims ? k.w : k.z) + 1;
+        v = make_int2(gb, s.ket[j0 + lane]);
+      }
+      for (unsigned long long rem = m64 | m32; rem;) {
+        const int kk = __builtin_amdgcn_readlane(key, __ffsll((long long)rem) - 1);
+        const unsigned long long same = __ballot(key == kk) & rem;
+        const unsigned long long s64 = same & m64, s32 = same & m32;
+        const int p64 = kw[2][kk], p32 = kw[3][kk];
+        if (same >> lane & 1ull) {
+          if (m64 >> lane & 1ull) s.list64[p64 + __popcll(s64 & below)] = v;
+          else s.list32[p32 + __popcll(s32 & below)] = v;
+        }
+        wave_sync();   // every lane has read the positions before lane 0 moves them
+        if (lane == 0) {
+          kw[2][kk] = p64 + __popcll(s64);
+          kw[3][kk] = p32 + __popcll(s32);
+        }
+        rem &= ~same;
+      }
+      wave_sync();   // the chunk's updates are seen by the next chunk's reads
     }
   }
 }
