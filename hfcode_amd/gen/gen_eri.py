@@ -1691,8 +1691,15 @@ KPASS_TWO = {("dldl", "double"), ("ddll", "double"), ("dlll", "float")}
 # pair: (dl|ll) 1431, (dd|dl) 831, (ll|ll) 803, (dl|dl) 385): there (dd|dl)
 # still gains (4.4 / 5.2 ms) while (ll|ll) loses (7.7 / 7.1), so the minimum
 # mean run is per class (DIG1_MIN_RUN, default 1024 quartets).
-DIG1_USE = {"dlll": 3, "llll": 3, "ddll": 3, "sdll": 3, "sddl": 3, "dldl": 1, "dddl": 1, "dddd": 2, "sddd": 2}
-DIG1_MIN_RUN = {"dddl": 512}
+# Re-measured with the wave sums of X_ab on DPP (no LDS shuffles) and global
+# block loads, one-pass / table: (sl|dl) FP64 build 5.9 / 6.5, mixed FP32
+# 3.02 / 3.38; (sl|dd) FP64 1.59 / 1.81, mixed FP64 / FP32 0.42 / 0.53,
+# 0.85 / 1.14 (mean runs 244 / 308); (dd|dd) FP32 0.91 / 1.10 (mean run
+# 193); (sd|dl) mixed FP64 0.35 / 0.40 (mean run 841); (sl|ll) still loses
+# (8.55 / 8.33, 4.73 / 4.58).
+DIG1_USE = {"dlll": 3, "llll": 3, "ddll": 3, "sdll": 3, "sddl": 3, "dldl": 1, "dddl": 1, "dddd": 2, "sddd": 2,
+            "sldl": 3, "sldd": 3}
+DIG1_MIN_RUN = {"dddl": 512, "sldl": 512, "sddl": 512, "sldd": 128, "dddd": 128, "sddd": 128}
 # one-pass digestion: next slice's loads issued before the current slice's
 # contractions, for the FP32 form and the FP64 form of the classes listed
 # (measured, C60 builds, ms: FP32 (dl|ll) 12.3 -> 11.9, (ll|ll) 9.3 -> 9.1,
