@@ -35,8 +35,8 @@ const double kShthresh = 1.0e-11;  // shthresh = intthresh/10 (molprops.f90:163)
 const double kNegab = 1.0e-12;     // extrathresh = intthresh*0.01 (precalc_kabs.f90:13)
 // Primitives per shell, at most: the reference's compile-time maxprim
 // (molprops.f90:9-18).  k_screen keeps the set of its survivors' ket
-// primitive-pair counts + 1 in two 64-bit words, which needs
-// kMaxShellPrims^2 + 1 < 128.
+// primitive-pair counts + 1 in two 64-bit words and counts them in
+// 128-entry LDS tables, which needs kMaxShellPrims^2 + 1 < 128.
 const int kMaxShellPrims = 10;
 static_assert(kMaxShellPrims * kMaxShellPrims + 1 < 128, "k_screen's primitive-count set holds keys below 128");
 // Screening statistics per class and list: surviving quartets, primitive
@@ -377,6 +377,7 @@ struct hfx_ctx {
   int dig_occ[3][HFX_NCLASSES] = {};   // resident digestion workgroups per CU [fp64, fp64 J-only, fp32]
   int digest1 = 1;                     // HFX_OPT_DIGEST: one-pass digestion where its LDS rows fit
   int dig1_occ[2][HFX_NCLASSES] = {};  // resident one-pass digestion workgroups per CU [fp64, fp32]
+  std::vector<double> diag_cost;   // diagonal builds: estimated work per pair type (stream balancing)
   hipEvent_t cev[2][HFX_NCLASSES][2] = {};
   int last_kind = 0;  // 0 none, 1 non-diag, 2 diag
   // device
@@ -633,10 +634,41 @@ int ensure_side_streams(hfx_ctx* c) {
 // list and, where the workgroup's LDS holds the rows of X that a bra pair's
 // quartets update (rows of shells A and B: J_ab, K_ac, K_ad, K_bc, K_bd),
 // accumulates them there and flushes them with row-coalesced atomics.
+// The integral buffer and vmax grow to `need` doubles (capped by the
+// context's limit; vmax to a sixteenth of that in rows).  Growing frees the
+// old buffers (an implicit device synchronisation), so they grow at least
+// twofold.
+int grow_eblk(hfx_ctx* c, long long need) {
+  const long long limit = live_contexts.load() == 1 ? kEblkDoublesAlone : kEblkDoubles;
+  need = std::min(limit, need);
+  if (need <= c->eblk_doubles) return HFX_OK;
+  if (c->capturing) return HFX_E_STATE;   // buffers are sized by an uncaptured build first
+  const long long cap = std::min(limit, std::max(need, 2 * c->eblk_doubles));
+  const long long rows = std::min(kMaxChunk, (cap / 16 + HFX_NT - 1) / HFX_NT * HFX_NT);
+  HIPOK(sync_all_streams(c));   // earlier chunks may still use the old buffers
+  drop_graphs(c);
+  dfree(c->d_eblk);
+  dfree(c->d_vmax);
+  c->eblk_doubles = c->vmax_rows = 0;
+  if (dalloc(&c->d_eblk, (size_t)cap) || dalloc(&c->d_vmax, (size_t)rows * kMaxCombo)) return HFX_E_NOMEM;
+  c->eblk_doubles = cap;
+  c->vmax_rows = rows;
+  return HFX_OK;
+}
+
+// A block class's own stream and share of the integral buffer and vmax, for
+// block classes that run side by side (the diagonal builds): the class runs
+// in one chunk of `rows` quartets, all its kernels on `st`.
+struct BlockSlot {
+  hipStream_t st;
+  long long eoff, voff, rows;   // doubles into d_eblk, doubles into d_vmax, quartets
+};
+
 int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long long n, double* dump,
-                 long long runs = 0) {
+                 long long runs = 0, const BlockSlot* slot = nullptr) {
   const ClassKernels& k = kClassKernels[cls];
   if (n <= 0) return HFX_OK;
+  hipStream_t ms = slot ? slot->st : c->stream;   // the class's stream (digestion, profile events)
   if (prec == 1 && a.diag) {   // the diagonal builds' element cut (k_cut_small) is on FP64 blocks
     fprintf(stderr, "hfx: FP32 blocks in a diagonal build\n");
     return HFX_E_STATE;
@@ -649,23 +681,20 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   // larger lists are chunked.  Growing frees the old buffer (an implicit
   // device synchronisation), so it grows at least twofold.
   const long long limit = live_contexts.load() == 1 ? kEblkDoublesAlone : kEblkDoubles;
-  const long long need = std::min(limit, ((n + HFX_NT - 1) / HFX_NT) * HFX_NT * ne);
-  if (need > c->eblk_doubles) {
-    if (c->capturing) return HFX_E_STATE;   // buffers are sized by an uncaptured build first
-    const long long cap = std::min(limit, std::max(need, 2 * c->eblk_doubles));
-    const long long rows = std::min(kMaxChunk, (cap / 16 + HFX_NT - 1) / HFX_NT * HFX_NT);
-    HIPOK(sync_all_streams(c));   // earlier chunks may still use the old buffers
-    drop_graphs(c);
-    dfree(c->d_eblk);
-    dfree(c->d_vmax);
-    c->eblk_doubles = c->vmax_rows = 0;
-    if (dalloc(&c->d_eblk, (size_t)cap) || dalloc(&c->d_vmax, (size_t)rows * kMaxCombo)) return HFX_E_NOMEM;
-    c->eblk_doubles = cap;
-    c->vmax_rows = rows;
+  if (slot) {   // the caller sized the buffers for every slot
+    const int nc = prec == 1 ? k.ncombo32 : k.ncombo;
+    if (slot->rows < n || slot->rows % HFX_NT || slot->eoff + slot->rows * ne > c->eblk_doubles ||
+        slot->voff + nc * slot->rows > c->vmax_rows * kMaxCombo || (prec == 1 ? k.cbuf32 : k.cbuf64) > 0) {
+      fprintf(stderr, "hfx: block slot does not fit the integral buffer\n");
+      return HFX_E_STATE;
+    }
+  } else {
+    int r = grow_eblk(c, ((n + HFX_NT - 1) / HFX_NT) * HFX_NT * ne);
+    if (r) return r;
   }
-  long long chunk = std::min(c->vmax_rows, (c->eblk_doubles / ne) / HFX_NT * HFX_NT);
-  if (c->chunk_opt > 0) chunk = std::min(chunk, c->chunk_opt);
-  if (c->cls_chunk[cls] > 0) chunk = std::min(chunk, c->cls_chunk[cls]);   // tuned quartets per chunk
+  long long chunk = slot ? slot->rows : std::min(c->vmax_rows, (c->eblk_doubles / ne) / HFX_NT * HFX_NT);
+  if (!slot && c->chunk_opt > 0) chunk = std::min(chunk, c->chunk_opt);
+  if (!slot && c->cls_chunk[cls] > 0) chunk = std::min(chunk, c->cls_chunk[cls]);   // tuned quartets per chunk
   // Several chunks, one live context: the buffers split into two halves and
   // the chunks alternate between them; the combination kernels run on the
   // side streams, the digestion on the context's stream, so chunk j's
@@ -674,7 +703,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   const long long half_e = c->eblk_doubles / 2 / 64 * 64;   // doubles per half, 512-byte aligned
   const long long half_v = c->vmax_rows / 2 * kMaxCombo;    // vmax doubles per half
   const long long half_chunk = std::min(c->vmax_rows / 2, half_e / ne) / HFX_NT * HFX_NT;
-  const bool pipelined = n > chunk && half_chunk >= HFX_NT && live_contexts.load() == 1 && !c->device_chunks;
+  const bool pipelined = !slot && n > chunk && half_chunk >= HFX_NT && live_contexts.load() == 1 && !c->device_chunks;
   if (pipelined) {
     chunk = std::min(chunk, half_chunk);
     int r = ensure_side_streams(c);
@@ -703,8 +732,10 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   }
   // every chunk's blocks and partial maxima stay inside its half (or the
   // whole buffer) of the integral buffer and of vmax
-  if (chunk < HFX_NT || chunk * ne > (pipelined ? half_e : c->eblk_doubles) ||
-      chunk > (pipelined ? c->vmax_rows / 2 : c->vmax_rows) || k.ncombo > kMaxCombo || k.ncombo32 > kMaxCombo) {
+  const bool fits = slot ? chunk >= HFX_NT
+                         : chunk >= HFX_NT && chunk * ne <= (pipelined ? half_e : c->eblk_doubles) &&
+                               chunk <= (pipelined ? c->vmax_rows / 2 : c->vmax_rows);
+  if (!fits || k.ncombo > kMaxCombo || k.ncombo32 > kMaxCombo) {
     fprintf(stderr, "hfx: chunk of %lld quartets does not fit the integral buffer\n", chunk);
     return HFX_E_STATE;
   }
@@ -736,7 +767,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
                         a.schw_out == nullptr && k.dig1 != nullptr && one_shm <= c->max_shm;
   const int ncombo = prec == 1 ? k.ncombo32 : k.ncombo;
   const bool prof = c->profile && !c->device_chunks;
-  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->stream));
+  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][0], ms));
   if (pipelined) {
     // the side streams start after everything already on the context's stream
     HIPOK(hipEventRecord(c->fork, c->stream));
@@ -745,8 +776,8 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   for (long long q0 = 0, j = 0; q0 < n; q0 += chunk, ++j) {
     const long long q1 = std::min(n, q0 + chunk);
     const int h = pipelined ? (int)(j & 1) : 0;
-    double* eblk = c->d_eblk + h * half_e;
-    double* vmax = c->d_vmax + h * half_v;
+    double* eblk = slot ? c->d_eblk + slot->eoff : c->d_eblk + h * half_e;
+    double* vmax = slot ? c->d_vmax + slot->voff : c->d_vmax + h * half_v;
     ClassArgs b = a;
     b.eblk = eblk;
     b.vmax = vmax;
@@ -766,7 +797,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
       nside = std::min(ncombo, (int)hfx_ctx::kSide);
       if (j >= 2)
         for (int s = 0; s < nside; ++s) HIPOK(hipStreamWaitEvent(c->side[s], c->done[h], 0));
-    } else if (ncombo > 1 && (q1 - q0 >= kConcurrentChunk || a.diag) && live_contexts.load() == 1 &&
+    } else if (!slot && ncombo > 1 && (q1 - q0 >= kConcurrentChunk || a.diag) && live_contexts.load() == 1 &&
                !c->device_chunks) {
       // (the diagonal builds' chunks are small but their kernels few and latency-bound)
       nside = std::min(ncombo - 1, (int)hfx_ctx::kSide);
@@ -790,7 +821,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
       for (int s = 1; s < nstream; ++s)
         if (load[s] < load[sb]) sb = s;
       load[sb] += cost[ic];
-      hipStream_t st = pipelined ? c->side[sb] : (sb == 0 ? c->stream : c->side[sb - 1]);
+      hipStream_t st = pipelined ? c->side[sb] : (sb == 0 ? ms : c->side[sb - 1]);
       const int per_cu = c->cls_per_cu[cls] > 0 ? c->cls_per_cu[cls] : 8;   // tuned launch width
       const dim3 grid(grid_for(q1 - q0, HFX_NT, c->ncu * per_cu));
       // a split combination's pre-kernels first, on the same stream
@@ -801,11 +832,11 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     }
     for (int s = 0; s < nside; ++s) {
       HIPOK(hipEventRecord(c->join[s], c->side[s]));
-      HIPOK(hipStreamWaitEvent(c->stream, c->join[s], 0));
+      HIPOK(hipStreamWaitEvent(ms, c->join[s], 0));
     }
     if (a.diag) {   // the diagonal builds' element cut, on the chunk's whole tiles
       const long long nel = (q1 - q0 + 63) / 64 * 64 * ne;
-      hipLaunchKernelGGL(k_cut_small, dim3(grid_for(nel, 256, 2048)), dim3(256), 0, c->stream, eblk, nel);
+      hipLaunchKernelGGL(k_cut_small, dim3(grid_for(nel, 256, 2048)), dim3(256), 0, ms, eblk, nel);
     }
     DigestArgs g;
     memset(&g, 0, sizeof(g));
@@ -854,7 +885,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
         }
         occ1 = std::min(nblk, 4);
       }
-      HIPOK(hipLaunchKernel(fn1, dim3(c->ncu * occ1), dim3(one_nt), gargs, one_shm, c->stream));
+      HIPOK(hipLaunchKernel(fn1, dim3(c->ncu * occ1), dim3(one_nt), gargs, one_shm, ms));
       if (pipelined) HIPOK(hipEventRecord(c->done[h], c->stream));
       continue;
     }
@@ -872,10 +903,10 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
       occ = std::min(nblk, 4);
     }
     const int grid = rows ? grid_for(q1 - q0, HFX_NT, c->ncu * occ) : grid_for(q1 - q0, HFX_NT, c->ncu * 8);
-    HIPOK(hipLaunchKernel(dfn, dim3(grid), dim3(HFX_NT), gargs, dig_shm, c->stream));
+    HIPOK(hipLaunchKernel(dfn, dim3(grid), dim3(HFX_NT), gargs, dig_shm, ms));
     if (pipelined) HIPOK(hipEventRecord(c->done[h], c->stream));
   }
-  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][1], c->stream));
+  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][1], ms));
   c->class_ran[prec][cls] = 1;
   return HFX_OK;
 }
@@ -1071,23 +1102,74 @@ int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initia
   if (!c->capturing) HIPOK(hipEventRecord(c->ev0, c->stream));
   if ((r = zero_build_state(c))) return r;
   if ((r = launch_maxdm(c, dP))) return r;
-  // The pair types' screens and classes are small, latency-bound launches:
-  // with one live context and launches sized on the host, the fused classes
-  // (no integral buffer) run on the side streams beside the block classes on
-  // the context's stream, and join before X is folded into F.
-  const bool conc = live_contexts.load() == 1 && !c->device_chunks && !c->capturing;
-  int nside_used = 0;
+  // The pair types' screens and classes are small, latency-bound launches
+  // (a few thousand quartets, one thread each): with one live context and
+  // launches sized on the host, each pair type's screen and class go to the
+  // least-loaded of the context's stream and the side streams, longest
+  // first.  A class's time is taken as its longest thread's operations (the
+  // pair with the most primitive pairs: np^2 primitive quartets, np bra
+  // primitives, one quartet), x1.5 for block classes (combination kernels
+  // one after another, then the digestion).  The block classes each get
+  // their own share of the integral buffer, and all join before X is folded
+  // into F.
+  const bool conc = live_contexts.load() == 1 && !c->device_chunks && !c->capturing && !c->debug_blocks;
+  hipStream_t streams[1 + hfx_ctx::kSide] = {c->stream};
+  int nstream = 1;
   if (conc) {
     if ((r = ensure_side_streams(c))) return r;
     HIPOK(hipEventRecord(c->fork, c->stream));
-    for (int k = 0; k < hfx_ctx::kSide; ++k) HIPOK(hipStreamWaitEvent(c->side[k], c->fork, 0));
+    for (int k = 0; k < hfx_ctx::kSide; ++k) {
+      HIPOK(hipStreamWaitEvent(c->side[k], c->fork, 0));
+      streams[nstream++] = c->side[k];
+    }
   }
-  for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
+  if (c->diag_cost.empty()) {   // per pair type, from the pairs' primitive counts
+    c->diag_cost.assign(HFX_NPAIRTYPES, 0.0);
+    for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
+      const ClassKernels& k = kClassKernels[kDiagClass[t]];
+      for (int g = c->offset[t]; g < c->offset[t] + c->npair[t]; ++g) {
+        const double np = k.all_prims ? c->pairs[g].npall : c->pairs[g].np;
+        const double ops = (np * np * k.flops_prim + np * k.flops_braprim + k.flops_quartet) * (k.split ? 1.5 : 1.0);
+        c->diag_cost[t] = std::max(c->diag_cost[t], ops);
+      }
+    }
+  }
+  int order[HFX_NPAIRTYPES];
+  for (int t = 0; t < HFX_NPAIRTYPES; ++t) order[t] = t;
+  if (conc)   // (else the pair types' order, as every context's build keeps it)
+    std::stable_sort(order, order + HFX_NPAIRTYPES, [&](int x, int y) { return c->diag_cost[x] > c->diag_cost[y]; });
+  // block classes side by side: one chunk each, in consecutive shares of the buffers
+  BlockSlot slots[HFX_NPAIRTYPES];
+  bool slotted = conc;
+  if (conc) {
+    long long eoff = 0, voff = 0;
+    for (int t = 0; t < HFX_NPAIRTYPES; ++t) {
+      const ClassKernels& k = kClassKernels[kDiagClass[t]];
+      if (c->npair[t] == 0 || !k.split) continue;
+      const long long ne = (long long)k.dims[0] * k.dims[1] * k.dims[2] * k.dims[3];
+      const long long rows = (c->npair[t] + HFX_NT - 1) / HFX_NT * HFX_NT;
+      slots[t] = BlockSlot{c->stream, eoff, voff, rows};
+      eoff += rows * ne;
+      voff += (long long)k.ncombo * rows;
+    }
+    if ((r = grow_eblk(c, eoff))) return r;
+    // (the integral buffer's cap: one class at a time in chunks instead)
+    slotted = eoff <= c->eblk_doubles && voff <= c->vmax_rows * kMaxCombo;
+  }
+  double load[1 + hfx_ctx::kSide] = {};
+  for (int io = 0; io < HFX_NPAIRTYPES; ++io) {
+    const int t = order[io];
     const int n = c->npair[t];
     if (n == 0) continue;
     const int cls = kDiagClass[t];
     hipStream_t st = c->stream;
-    if (conc && !kClassKernels[cls].split && !c->debug_blocks) st = c->side[nside_used++ % hfx_ctx::kSide];
+    if (conc && (slotted || !kClassKernels[cls].split)) {
+      int sb = 0;
+      for (int k = 1; k < nstream; ++k)
+        if (load[k] < load[sb]) sb = k;
+      load[sb] += c->diag_cost[t];
+      st = streams[sb];
+    }
     DiagScreenArgs s;
     memset(&s, 0, sizeof(s));
     s.pairs = c->d_pairs;
@@ -1121,10 +1203,14 @@ int build_diag(hfx_ctx* c, const double* dP, double* dF, double thr, bool initia
         a.blocks = cp->blocks64;
       }
     }
-    if (kClassKernels[cls].split)
+    if (kClassKernels[cls].split && conc && slotted) {
+      slots[t].st = st;
+      r = launch_split(c, cls, 0, false, a, n, a.blocks, 0, &slots[t]);
+    } else if (kClassKernels[cls].split) {
       r = launch_split(c, cls, 0, false, a, n, a.blocks);
-    else
+    } else {
       r = launch_class(c, cls, 0, false, a, n, st);
+    }
     if (r) return r;
   }
   if (conc) {
