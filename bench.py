@@ -233,12 +233,13 @@ def run_batch(a):
             peak[0] = max(peak[0], u)
         if done % 64 == 0 or done == total:
             log("batch: %d/%d solved, %.1f s" % (done, total, el))
+    maxiter = 100
     if a.path == "batched":
         res, wall = batch.run_batched(mols, "def2-sv(p)", group=a.group, workers=a.workers, split=split or None,
-                                      progress=progress)
+                                      progress=progress, maxiter=maxiter)
     else:
         res, wall = batch.run_batch(mols, "def2-sv(p)", workers=a.workers, progress=progress, split=split or None,
-                                    device_scf=a.scf == "device")
+                                    device_scf=a.scf == "device", maxiter=maxiter)
     nbs = [r["nb"] for r in res]
     its = [r["iterations"] for r in res]
     conv = sum(r["converged"] for r in res)
@@ -254,7 +255,7 @@ def run_batch(a):
         "dtype": "f64+f32" if split else "f64",
         "data": "synthetic: all-trans alkanes / alkanols / amines, 3-20 heavy atoms, "
                 "ideal bonds + 0.05 A Gaussian jitter, seed 2024",
-        "config": {"workload": "batched RHF solves (core guess, DIIS from iteration 2, thr 1e-10, <= 50 iterations)",
+        "config": {"workload": "batched RHF solves (core guess, DIIS from iteration 2, thr 1e-10, <= %d iterations)" % maxiter,
                    "basis": "def2-SV(P)", "molecules": len(mols), "nb_min": min(nbs), "nb_max": max(nbs),
                    "nb_mean": round(float(np.mean(nbs)), 1), "path": a.path, "group": a.group,
                    "workers": a.workers, "mixed_split": split, "parallelism": "single GPU"},

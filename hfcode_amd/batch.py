@@ -275,13 +275,14 @@ def group_by_size(nbs, group):
     return [order[k:k + group] for k in range(0, len(order), group)]
 
 
-def run_batched(mols, basis_name, group=None, workers=2, maxiter=64, split=None, use_graphs=True, progress=None):
+def run_batched(mols, basis_name, group=None, workers=2, maxiter=100, split=None, use_graphs=True, progress=None):
     """Solve every molecule of `mols` in size-sorted groups of `group`
     molecules (default: the tuned group size of batch_launch.json, else 64),
     `workers` groups at once; returns (results in input order, wall seconds).
-    At most 64 iterations: from the core guess the reference's DIIS schedule
+    At most 100 iterations: from the core guess the reference's DIIS schedule
     takes 50 iterations for some long chains even with the reference's own
-    Fock builds."""
+    Fock builds, and one of the 512 def2-SV(P) chains of the batched workload
+    (a C19H40, nb 346) needs 64 in FP64."""
     fock.lib()
     if group is None:
         from .gen import autotune_batch
