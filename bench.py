@@ -389,6 +389,12 @@ def main():
     fb.set_mixed(0.0)
     # one FP64 build at the tighter cut-off the reference configuration also uses
     dt12, _, _, st12 = time_builds(fb, dP, dH, dF, 1e-12, 1, 1, torch)
+    # class spans with the block-class lists one after another (untimed): in
+    # the timed builds a class's first and last kernels overlap its
+    # neighbours' digestion and integrals, which its span then includes
+    fb.set_overlap(False)
+    _, cms_alone, _, _ = time_builds(fb, dP, dH, dF, a.thr, 1, 1, torch)
+    fb.set_overlap(True)
     err = abs(Emx - E64)
     b64 = a.steps / dt64
     bmx = a.steps / dtmx
@@ -402,6 +408,7 @@ def main():
     # quartets the screening passed.
     k = int(np.argmax(cms64[:, 0]))
     ach = cfl64[k, 0] / (cms64[k, 0] * 1e-3) / 1e12
+    ach_alone = cfl64[k, 0] / (cms_alone[k, 0] * 1e-3) / 1e12
     cname = fock.CLASSES[k]
     kinfo = _class_kernels(cname)
     tot_fl = st64["flops_fp64"]
@@ -444,6 +451,11 @@ def main():
             "traffic": None,
             "flops_per_launch": cfl64[k, 0],
             "avg_launch_ms": round(float(cms64[k, 0]), 4),
+            "span_note": "the class's span in the timed builds, first to last of its launches; its first and "
+                         "last kernels run beside the neighbouring lists' kernels",
+            "alone": {"avg_launch_ms": round(float(cms_alone[k, 0]), 4), "achieved": round(ach_alone, 3),
+                      "frac": round(ach_alone / PEAK_FP64_TFLOPS, 4),
+                      "note": "one untimed FP64 build with the block-class lists one after another"},
         },
         "class_ms_fp64": {fock.CLASSES[i]: round(float(cms64[i, 0]), 3) for i in range(46) if cms64[i, 0] > 0},
         "class_ms_mixed": {fock.CLASSES[i]: [round(float(cmsmx[i, 0]), 3), round(float(cmsmx[i, 1]), 3)]

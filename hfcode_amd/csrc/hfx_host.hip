@@ -392,6 +392,14 @@ struct hfx_ctx {
   hipStream_t side[kSide] = {};
   hipEvent_t fork = nullptr, join[kSide] = {};
   hipEvent_t done[2] = {};        // digestion of the last chunk in each half of the integral buffer
+  // a build's block-class lists pipelined across lists (launch_split): on,
+  // chunks so far, halves used, and the point after which the first chunks'
+  // combination kernels may start
+  bool overlap = true;   // HFX_OPT_OVERLAP
+  bool xp = false;
+  long long xp_j = 0;
+  bool xp_used[2] = {false, false};
+  hipEvent_t xp_start = nullptr;
   PairRec* d_pairs = nullptr;
   PrimRec* d_prims = nullptr;
   PrimRec32* d_prims32 = nullptr;
@@ -619,6 +627,7 @@ int ensure_side_streams(hfx_ctx* c) {
   }
   for (int h = 0; h < 2; ++h)
     if (!c->done[h]) HIPOK(hipEventCreateWithFlags(&c->done[h], hipEventDisableTiming));
+  if (!c->xp_start) HIPOK(hipEventCreateWithFlags(&c->xp_start, hipEventDisableTiming));
   HIPOK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
   return HFX_OK;
 }
@@ -653,6 +662,23 @@ int grow_eblk(hfx_ctx* c, long long need) {
   if (dalloc(&c->d_eblk, (size_t)cap) || dalloc(&c->d_vmax, (size_t)rows * kMaxCombo)) return HFX_E_NOMEM;
   c->eblk_doubles = cap;
   c->vmax_rows = rows;
+  return HFX_OK;
+}
+
+// After a chunk's digestion (on the context's stream): the event that frees
+// its half of the integral buffer; across lists also the half's use and the
+// chunk count, and a list that used the whole buffer frees both halves.
+int after_digestion(hfx_ctx* c, bool pipelined, bool xp, int h) {
+  if (pipelined) HIPOK(hipEventRecord(c->done[h], c->stream));
+  if (xp) {
+    c->xp_used[h] = true;
+    ++c->xp_j;
+  } else if (c->xp && c->done[0]) {
+    for (int k = 0; k < 2; ++k) {
+      HIPOK(hipEventRecord(c->done[k], c->stream));
+      c->xp_used[k] = true;
+    }
+  }
   return HFX_OK;
 }
 
@@ -703,7 +729,14 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   const long long half_e = c->eblk_doubles / 2 / 64 * 64;   // doubles per half, 512-byte aligned
   const long long half_v = c->vmax_rows / 2 * kMaxCombo;    // vmax doubles per half
   const long long half_chunk = std::min(c->vmax_rows / 2, half_e / ne) / HFX_NT * HFX_NT;
-  const bool pipelined = !slot && n > chunk && half_chunk >= HFX_NT && live_contexts.load() == 1 && !c->device_chunks;
+  // Across lists (c->xp, one build's block classes with one live context):
+  // every list runs in half-buffer chunks, the chunk count continuing from
+  // the previous list, so a list's first combination kernels wait only for
+  // the digestion that last read their half, not for the previous list's
+  // last digestion.
+  const bool xp = c->xp && !slot && !a.diag && dump == nullptr && half_chunk >= HFX_NT;
+  const bool pipelined = xp || (!slot && n > chunk && half_chunk >= HFX_NT && live_contexts.load() == 1 &&
+                                !c->device_chunks);
   if (pipelined) {
     chunk = std::min(chunk, half_chunk);
     int r = ensure_side_streams(c);
@@ -767,15 +800,15 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
                         a.schw_out == nullptr && k.dig1 != nullptr && one_shm <= c->max_shm;
   const int ncombo = prec == 1 ? k.ncombo32 : k.ncombo;
   const bool prof = c->profile && !c->device_chunks;
-  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][0], ms));
-  if (pipelined) {
+  if (prof && !xp) HIPOK(hipEventRecord(c->cev[prec][cls][0], ms));
+  if (pipelined && !xp) {
     // the side streams start after everything already on the context's stream
     HIPOK(hipEventRecord(c->fork, c->stream));
     for (int s = 0; s < hfx_ctx::kSide; ++s) HIPOK(hipStreamWaitEvent(c->side[s], c->fork, 0));
   }
   for (long long q0 = 0, j = 0; q0 < n; q0 += chunk, ++j) {
     const long long q1 = std::min(n, q0 + chunk);
-    const int h = pipelined ? (int)(j & 1) : 0;
+    const int h = xp ? (int)(c->xp_j & 1) : pipelined ? (int)(j & 1) : 0;
     double* eblk = slot ? c->d_eblk + slot->eoff : c->d_eblk + h * half_e;
     double* vmax = slot ? c->d_vmax + slot->voff : c->d_vmax + h * half_v;
     ClassArgs b = a;
@@ -795,7 +828,11 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     int nside = 0;
     if (pipelined) {
       nside = std::min(ncombo, (int)hfx_ctx::kSide);
-      if (j >= 2)
+      if (xp) {   // the half's last reader, or the build's start
+        for (int s = 0; s < nside; ++s)
+          HIPOK(hipStreamWaitEvent(c->side[s], c->xp_used[h] ? c->done[h] : c->xp_start, 0));
+        if (prof && j == 0) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->side[0]));   // the class's first kernel
+      } else if (j >= 2)
         for (int s = 0; s < nside; ++s) HIPOK(hipStreamWaitEvent(c->side[s], c->done[h], 0));
     } else if (!slot && ncombo > 1 && (q1 - q0 >= kConcurrentChunk || a.diag) && live_contexts.load() == 1 &&
                !c->device_chunks) {
@@ -886,7 +923,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
         occ1 = std::min(nblk, 4);
       }
       HIPOK(hipLaunchKernel(fn1, dim3(c->ncu * occ1), dim3(one_nt), gargs, one_shm, ms));
-      if (pipelined) HIPOK(hipEventRecord(c->done[h], c->stream));
+      if (int r2 = after_digestion(c, pipelined, xp, h)) return r2;
       continue;
     }
     const void* dfn = prec == 1 ? (multi ? k.digfm : k.digf) : (jonly ? (multi ? k.digjm : k.digj) : (multi ? k.digm : k.dig));
@@ -904,7 +941,7 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     }
     const int grid = rows ? grid_for(q1 - q0, HFX_NT, c->ncu * occ) : grid_for(q1 - q0, HFX_NT, c->ncu * 8);
     HIPOK(hipLaunchKernel(dfn, dim3(grid), dim3(HFX_NT), gargs, dig_shm, ms));
-    if (pipelined) HIPOK(hipEventRecord(c->done[h], c->stream));
+    if (int r2 = after_digestion(c, pipelined, xp, h)) return r2;
   }
   if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][1], ms));
   c->class_ran[prec][cls] = 1;
@@ -1062,6 +1099,20 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
     HIPOK(hipMemcpyAsync(hcounts, c->d_counts, sizeof(hcounts), hipMemcpyDeviceToHost, c->stream));
     HIPOK(hipMemcpyAsync(hstats, c->d_stats, sizeof(hstats), hipMemcpyDeviceToHost, c->stream));
     HIPOK(hipStreamSynchronize(c->stream));
+  }
+  // block-class lists pipelined across lists (launch_split) while this
+  // build launches them: one live context, launches sized on the host
+  struct XpScope {
+    hfx_ctx* c;
+    ~XpScope() { c->xp = false; }
+  } xp_scope{c};
+  if (c->overlap && any_split && !c->device_chunks && !c->capturing && !c->debug_blocks && c->nmol == 1 &&
+      live_contexts.load() == 1) {
+    if ((r = ensure_side_streams(c))) return r;
+    c->xp = true;
+    c->xp_j = 0;
+    c->xp_used[0] = c->xp_used[1] = false;
+    HIPOK(hipEventRecord(c->xp_start, c->stream));   // lists, counts and P32 are ready
   }
   for (auto& cp : c->classes) {
     if (cp.cap <= 0) continue;
@@ -1602,6 +1653,7 @@ void hfx_ctx_destroy(hfx_ctx* c) {
   for (int h = 0; h < 2; ++h)
     if (c->done[h]) hipEventDestroy(c->done[h]);
   if (c->fork) hipEventDestroy(c->fork);
+  if (c->xp_start) hipEventDestroy(c->xp_start);
   if (c->stream) hipStreamDestroy(c->stream);
   if (c->counted) live_contexts.fetch_sub(1);
   delete c;
@@ -1633,6 +1685,11 @@ int hfx_set_option(hfx_ctx* c, int key, double value) {
     if (!(value >= 0.0) || value > 1e12) return HFX_E_ARG;
     c->chunk_opt = (long long)value / HFX_NT * HFX_NT;
     if (value > 0.0 && c->chunk_opt == 0) c->chunk_opt = HFX_NT;
+    return HFX_OK;
+  }
+  if (key == HFX_OPT_OVERLAP) {
+    if (value != 0.0 && value != 1.0) return HFX_E_ARG;
+    c->overlap = value != 0.0;
     return HFX_OK;
   }
   if (key == HFX_OPT_DIGEST) {

@@ -29,7 +29,7 @@ CLASSES = ("ssss sssp sssd sssl sspp sspd sspl ssdd ssdl ssll spsp spsd spsl spp
 NTYPES = 10
 NCLASSES = 46
 HFX_JK, HFX_J_ONLY = 0, 1
-OPT_MIXED_SPLIT, OPT_DEBUG_BLOCKS, OPT_PROFILE, OPT_CHUNK, OPT_DIGEST = 1, 2, 3, 4, 5
+OPT_MIXED_SPLIT, OPT_DEBUG_BLOCKS, OPT_PROFILE, OPT_CHUNK, OPT_DIGEST, OPT_OVERLAP = 1, 2, 3, 4, 5, 6
 ERRORS = {-1: "bad argument", -2: "unsupported basis", -3: "HIP runtime error",
           -4: "device out of memory", -5: "Schwarz tables missing (run initial_diag first)"}
 
@@ -364,6 +364,12 @@ class FockBuilder:
         block, rows of X and P in LDS, where it measured faster and bra runs
         are long (default); 2 wherever it fits; 0 (or False) multi-pass."""
         _check(lib().hfx_set_option(self._h, OPT_DIGEST, float(int(mode))))
+
+    def set_overlap(self, on=True):
+        """Block-class lists of a build: each list's first integral kernels
+        beside the previous list's last digestion (default), or one list
+        after another; same results."""
+        _check(lib().hfx_set_option(self._h, OPT_OVERLAP, 1.0 if on else 0.0))
 
     def _prep(self, f, p):
         nb = self.nb

@@ -42,6 +42,10 @@ extern "C" {
                                    where that measured faster and the bra pairs' runs of quartets are
                                    long (default); 2 = wherever its LDS fits; 0 = never (multi-pass
                                    digestion).  Same sums, in another order. */
+#define HFX_OPT_OVERLAP 6       /* 1 (default): with one live context, a build's block-class lists run
+                                   in half-buffer chunks, each list's first integral kernels beside the
+                                   previous list's last digestion; 0: lists one after another.  A
+                                   scheduling knob: same results. */
 
 /* pair types, in the order of the reference's pair lists (molprops.f90:122-131) */
 #define HFX_NPAIRTYPES 10       /* ss sp sd sl pp pd pl dd dl ll */
