@@ -1114,7 +1114,35 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
     c->xp_used[0] = c->xp_used[1] = false;
     HIPOK(hipEventRecord(c->xp_start, c->stream));   // lists, counts and P32 are ready
   }
-  for (auto& cp : c->classes) {
+  // Class order: with lists overlapping, the fused classes (on the
+  // context's stream, like the digestion) go between the block classes,
+  // costliest first, so that each runs beside the next block class's
+  // integral kernels on the side streams rather than alone.
+  std::vector<int> corder;
+  for (int i = 0; i < (int)c->classes.size(); ++i) corder.push_back(i);
+  if (c->xp) {
+    std::vector<int> blk, fus;
+    std::vector<double> cost(c->classes.size(), 0.0);
+    for (int i = 0; i < (int)c->classes.size(); ++i) {
+      const int cl = c->classes[i].cls;
+      const ClassKernels& k = kClassKernels[cl];
+      for (int p = 0; p < 2; ++p) {
+        const unsigned long long* st = hstats + kStatN * (p * HFX_NCLASSES + cl);
+        cost[i] += (double)st[1] * k.flops_prim + (double)st[0] * k.flops_quartet;
+      }
+      (k.split ? blk : fus).push_back(i);
+    }
+    std::stable_sort(fus.begin(), fus.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    corder.clear();
+    size_t f = 0;
+    for (int i : blk) {
+      corder.push_back(i);
+      if (f < fus.size()) corder.push_back(fus[f++]);
+    }
+    while (f < fus.size()) corder.push_back(fus[f++]);
+  }
+  for (int ci : corder) {
+    ClassPlan& cp = c->classes[ci];
     if (cp.cap <= 0) continue;
     for (int prec = 0; prec < (mixed ? 2 : 1); ++prec) {
       ClassArgs a = base_args(c, dP, thr);

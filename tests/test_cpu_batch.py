@@ -107,6 +107,8 @@ def test_graph_replays_equal_uncaptured_builds(setup):
     n = bb.nmol
     P = bb.stack([a["P"] for a in alone])
     bb.initial_diag(np.zeros_like(P), P, 5e-7)
+    for a in alone:   # the same Schwarz tables in each molecule's own context (whatever ran before)
+        a["fb"].initial_diag(np.zeros_like(a["P"]), a["P"], 5e-7)
     Pd = np.zeros_like(P)
     Fg = np.zeros_like(P)
     Fe = np.zeros_like(P)
