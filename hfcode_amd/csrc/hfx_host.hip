@@ -378,7 +378,9 @@ struct hfx_ctx {
   int digest1 = 1;                     // HFX_OPT_DIGEST: one-pass digestion where its LDS rows fit
   int dig1_occ[2][HFX_NCLASSES] = {};  // resident one-pass digestion workgroups per CU [fp64, fp32]
   std::vector<double> diag_cost;   // diagonal builds: estimated work per pair type (stream balancing)
-  hipEvent_t cev[2][HFX_NCLASSES][2] = {};
+  // class start, end, and a second start (the other side stream when a
+  // class's first kernels start on both): HFX_OPT_PROFILE spans
+  hipEvent_t cev[2][HFX_NCLASSES][3] = {};
   int last_kind = 0;  // 0 none, 1 non-diag, 2 diag
   // device
   hipStream_t stream = nullptr;
@@ -587,7 +589,10 @@ int launch_class(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
   const ClassKernels& k = kClassKernels[cls];
   if (!st) st = c->stream;
   const bool prof = c->profile && !c->device_chunks;
-  if (prof) HIPOK(hipEventRecord(c->cev[prec][cls][0], st));
+  if (prof) {
+    HIPOK(hipEventRecord(c->cev[prec][cls][0], st));
+    HIPOK(hipEventRecord(c->cev[prec][cls][2], st));
+  }
   const void* fn = prec == 1 ? k.f32 : (jonly ? k.f64j : k.f64);
   const int K = prec == 1 ? 5 : 8;
   const size_t boys_shm = (size_t)HFX_BOYS_ROWS * (k.M + K) * (prec == 1 ? sizeof(float) : sizeof(double));
@@ -800,7 +805,10 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
                         a.schw_out == nullptr && k.dig1 != nullptr && one_shm <= c->max_shm;
   const int ncombo = prec == 1 ? k.ncombo32 : k.ncombo;
   const bool prof = c->profile && !c->device_chunks;
-  if (prof && !xp) HIPOK(hipEventRecord(c->cev[prec][cls][0], ms));
+  if (prof && !xp) {
+    HIPOK(hipEventRecord(c->cev[prec][cls][0], ms));
+    HIPOK(hipEventRecord(c->cev[prec][cls][2], ms));
+  }
   if (pipelined && !xp) {
     // the side streams start after everything already on the context's stream
     HIPOK(hipEventRecord(c->fork, c->stream));
@@ -831,7 +839,10 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
       if (xp) {   // the half's last reader, or the build's start
         for (int s = 0; s < nside; ++s)
           HIPOK(hipStreamWaitEvent(c->side[s], c->xp_used[h] ? c->done[h] : c->xp_start, 0));
-        if (prof && j == 0) HIPOK(hipEventRecord(c->cev[prec][cls][0], c->side[0]));   // the class's first kernel
+        if (prof && j == 0) {   // before the class's first kernels: the span runs from the earlier of the two
+          HIPOK(hipEventRecord(c->cev[prec][cls][0], c->side[0]));
+          HIPOK(hipEventRecord(c->cev[prec][cls][2], c->side[nside > 1 ? 1 : 0]));
+        }
       } else if (j >= 2)
         for (int s = 0; s < nside; ++s) HIPOK(hipStreamWaitEvent(c->side[s], c->done[h], 0));
     } else if (!slot && ncombo > 1 && (q1 - q0 >= kConcurrentChunk || a.diag) && live_contexts.load() == 1 &&
@@ -1672,7 +1683,7 @@ void hfx_ctx_destroy(hfx_ctx* c) {
   if (c->ev1) hipEventDestroy(c->ev1);
   for (int p = 0; p < 2; ++p)
     for (int k = 0; k < HFX_NCLASSES; ++k)
-      for (int e = 0; e < 2; ++e)
+      for (int e = 0; e < 3; ++e)
         if (c->cev[p][k][e]) hipEventDestroy(c->cev[p][k][e]);
   for (int s = 0; s < hfx_ctx::kSide; ++s) {
     if (c->side[s]) hipStreamDestroy(c->side[s]);
@@ -1730,7 +1741,7 @@ int hfx_set_option(hfx_ctx* c, int key, double value) {
     if (value != 0.0 && !c->cev[0][0][0]) {
       for (int p = 0; p < 2; ++p)
         for (int k = 0; k < HFX_NCLASSES; ++k)
-          for (int e = 0; e < 2; ++e) HIPOK(hipEventCreate(&c->cev[p][k][e]));
+          for (int e = 0; e < 3; ++e) HIPOK(hipEventCreate(&c->cev[p][k][e]));
     }
     c->profile = value != 0.0;
     return HFX_OK;
@@ -2122,9 +2133,10 @@ int hfx_get_stats(hfx_ctx* c, hfx_stats* out) {
       const double f = (double)s[1] * (p ? k.flops_prim_fp32 : k.flops_prim) + (double)s[2] * k.flops_braprim;
       out->class_flops[cl][p] = f + (double)s[0] * k.flops_quartet;
       if (c->profile && c->class_ran[p][cl]) {
-        float cms = 0.f;
+        float cms = 0.f, cms2 = 0.f;
         HIPOK(hipEventElapsedTime(&cms, c->cev[p][cl][0], c->cev[p][cl][1]));
-        out->class_ms[cl][p] = cms;
+        HIPOK(hipEventElapsedTime(&cms2, c->cev[p][cl][2], c->cev[p][cl][1]));
+        out->class_ms[cl][p] = std::max(cms, cms2);
       }
       if (p) {
         out->flops_fp32 += f;
