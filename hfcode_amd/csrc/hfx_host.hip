@@ -822,7 +822,8 @@ int launch_split(hfx_ctx* c, int cls, int prec, bool jonly, ClassArgs a, long lo
     ClassArgs b = a;
     b.eblk = eblk;
     b.vmax = vmax;
-    b.cbuf = cbn > 0 ? c->d_cbuf + h * cbn * chunk : nullptr;
+    // (halves at a fixed offset: lists of other classes alternate halves too)
+    b.cbuf = cbn > 0 ? c->d_cbuf + (size_t)h * (size_t)(c->cbuf_doubles / 2) : nullptr;
     b.vstride = (int)chunk;
     b.q0 = (int)q0;
     b.q1 = (int)q1;
