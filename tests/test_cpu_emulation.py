@@ -171,6 +171,27 @@ def test_one_pass_digestion_equals_multi_pass(emu_fock, split, chunk):
         assert np.abs(one - g["F_nd_1e-10"]).max() < TOL
 
 
+@pytest.mark.parametrize("split,chunk", [(0.0, 256), (1e-5, 512)])
+def test_overlapping_lists_equal_lists_in_turn(emu_fock, split, chunk):
+    """Block-class lists in half-buffer chunks numbered across the build
+    (HFX_OPT_OVERLAP 1, the default with one live context) against one list
+    after another: the same quartets, added in another order."""
+    g = Golden("benzene_631gd")
+    z = np.zeros((g.nb, g.nb))
+    fb = emu_fock.FockBuilder(g.shells, g.pairs, nb=g.nb)
+    fb.set_mixed(split)
+    fb.set_chunk(chunk)
+    fb.set_overlap(True)
+    over = [fb.non_diag(z, g.P, 1e-10) for _ in range(2)]
+    fb.set_overlap(False)
+    turn = fb.non_diag(z, g.P, 1e-10)
+    fb.close()
+    for F in over:
+        assert np.abs(F - turn).max() < 1e-12
+    if split == 0.0:
+        assert np.abs(turn - g["F_nd_1e-10"]).max() < TOL
+
+
 @pytest.mark.parametrize("name", ["water_sto3g", "water_def2svp"])
 def test_standalone_path_and_scf(emu_fock, name):
     """Own shells, one-electron matrices, Schwarz tables and SCF driver

@@ -59,6 +59,29 @@ def test_one_pass_digestion_equals_multi_pass(split, chunk):
         assert np.abs(one - g["F_nd_1e-10"]).max() < TOL
 
 
+@pytest.mark.parametrize("split,chunk", [(0.0, 256), (1e-5, 512), (0.0, 0)])
+def test_overlapping_lists_equal_lists_in_turn(split, chunk):
+    """Block-class lists in half-buffer chunks numbered across the build,
+    each list's first integral kernels beside the previous list's last
+    digestion (HFX_OPT_OVERLAP 1, the default with one live context),
+    against one list after another: same quartets, another order of the
+    additions; three overlapped builds in a row."""
+    print("overlap, split %g chunk %d" % (split, chunk), flush=True)
+    g = Golden("benzene_631gd")
+    z = np.zeros((g.nb, g.nb))
+    fb = builder(g)
+    fb.set_mixed(split)
+    fb.set_chunk(chunk)
+    fb.set_overlap(False)
+    turn = fb.non_diag(z, g.P, 1e-10)
+    fb.set_overlap(True)
+    for rep in range(3):
+        assert np.abs(fb.non_diag(z, g.P, 1e-10) - turn).max() < (1e-12 if split == 0.0 else 1e-9), rep
+    fb.close()
+    if split == 0.0:
+        assert np.abs(turn - g["F_nd_1e-10"]).max() < TOL
+
+
 @pytest.mark.parametrize("name", ["water_def2svp", "benzene_631gd"])
 def test_j_only_matches_oracle(name):
     import oracle
