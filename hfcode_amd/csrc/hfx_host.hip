@@ -1153,9 +1153,18 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
     }
     while (f < fus.size()) corder.push_back(fus[f++]);
   }
+  // with lists overlapping, every other fused class on a side stream (the
+  // context's stream carries every digestion)
+  int nfused = 0, side_fused = 0;
   for (int ci : corder) {
     ClassPlan& cp = c->classes[ci];
     if (cp.cap <= 0) continue;
+    hipStream_t fst = nullptr;
+    if (c->xp && !kClassKernels[cp.cls].split && (nfused++ & 1)) {
+      if (side_fused == 0)   // (after the build's set-up, whatever the block lists launched)
+        for (int k = 0; k < hfx_ctx::kSide; ++k) HIPOK(hipStreamWaitEvent(c->side[k], c->xp_start, 0));
+      fst = c->side[side_fused++ % hfx_ctx::kSide];
+    }
     for (int prec = 0; prec < (mixed ? 2 : 1); ++prec) {
       ClassArgs a = base_args(c, dP, thr);
       a.thr_mol = thr_mol;
@@ -1171,10 +1180,15 @@ int build_nd(hfx_ctx* c, const double* dP, double* dF, double thr, int mode, dou
                          c->device_chunks ? cp.cap : hcounts[prec * HFX_NCLASSES + cp.cls], a.blocks,
                          c->device_chunks ? 0 : (long long)hstats[kStatN * (prec * HFX_NCLASSES + cp.cls) + 3]);
       else
-        r = launch_class(c, cp.cls, prec, jonly, a, cp.cap);
+        r = launch_class(c, cp.cls, prec, jonly, a, cp.cap, fst);
       if (r) return r;
     }
   }
+  if (side_fused > 0)   // the side streams' fused classes before X is folded into F
+    for (int k = 0; k < hfx_ctx::kSide; ++k) {
+      HIPOK(hipEventRecord(c->join[k], c->side[k]));
+      HIPOK(hipStreamWaitEvent(c->stream, c->join[k], 0));
+    }
   hipLaunchKernelGGL(k_pairbuf_to_x, dim3(grid_for((long long)c->totpairs * HFX_PAIRBUF_STRIDE, 256, 2048)), dim3(256), 0,
                      c->stream, c->d_pairbuf, c->d_pairs, c->d_shells, c->totpairs, c->d_X, c->nb);
   hipLaunchKernelGGL(k_add_sym, dim3(grid_for((long long)c->nb * c->nb * c->nmol, 256, 4096)), dim3(256), 0, c->stream,
