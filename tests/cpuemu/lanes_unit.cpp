@@ -19,6 +19,8 @@
 //   (largest first) then ket position, counts and statistics equal;
 // * k_screen_diag (ballot compaction through wave_append): the list holds
 //   exactly the diagonal pairs a direct loop keeps, once each;
+// * wave_sum (DPP row scans + readlane): every lane of a wave gets the sum of
+//   the wave's 64 values, for each of four waves of a workgroup;
 // * pick_resident: the pair chosen is one of its candidates (the current
 //   pair, the step's first, middle and last quartets' bras) and none of them
 //   owns more of the step's quartets; the current pair is kept on a tie.
@@ -76,6 +78,32 @@ static bool check_collectives() {
     CHECK(out[4 * t + 2] == (unsigned long long)(2 * (w0 + 63 - lane)), "shfl of lane %d", t);
     CHECK(out[4 * t + 3] == 77, "syncthreads_count of lane %d", t);
   }
+  return true;
+}
+
+// ---- wave_sum -----------------------------------------------------------------
+__global__ void k_wave_sum(const double* in, double* out) { out[threadIdx.x] = hfx::wave_sum(in[threadIdx.x]); }
+
+static bool check_wave_sum(unsigned seed) {
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> u(-1.0, 1.0);
+  for (int rep = 0; rep < 4; ++rep) {
+    std::vector<double> in(256), out(256, -7.0);
+    for (int t = 0; t < 256; ++t)   // one wave of zeros but one lane, the others mixed magnitudes
+      in[t] = rep == 1 && t < 64 ? (t == 37 ? 3.0 : 0.0) : u(rng) * std::pow(10.0, (double)(rng() % 7) - 3.0);
+    hipLaunchKernelGGL(k_wave_sum, dim3(1), dim3(256), 0, 0, in.data(), out.data());
+    for (int w = 0; w < 4; ++w) {
+      double ref = 0.0, mag = 0.0;
+      for (int l = 0; l < 64; ++l) {
+        ref += in[64 * w + l];
+        mag += std::fabs(in[64 * w + l]);
+      }
+      for (int l = 0; l < 64; ++l)
+        CHECK(std::fabs(out[64 * w + l] - ref) <= 1e-14 * mag, "wave_sum rep %d wave %d lane %d: %.17g vs %.17g", rep,
+              w, l, out[64 * w + l], ref);
+    }
+  }
+  std::printf("ok wave_sum\n");
   return true;
 }
 
@@ -422,6 +450,7 @@ int main(int argc, char** argv) {
   run_diag_case(130, 0, 2, 7);
   run_diag_case(0, 1, 1, 8);
   check_pick_resident(9);
+  check_wave_sum(10);
   std::printf(g_fail ? "%d checks FAILED\n" : "all checks passed\n", g_fail);
   return g_fail ? 1 : 0;
 }

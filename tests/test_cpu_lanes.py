@@ -49,7 +49,7 @@ def test_screen_compaction_and_votes(lanes_unit):
     print(r.stdout)
     assert r.returncode == 0 and "all checks passed" in r.stdout
     assert r.stdout.count("ok k_screen ") == 10 and r.stdout.count("ok k_screen_diag") == 4
-    assert "ok pick_resident" in r.stdout and "ok collectives" in r.stdout
+    assert "ok pick_resident" in r.stdout and "ok collectives" in r.stdout and "ok wave_sum" in r.stdout
 
 
 def test_divergent_collective_is_reported(lanes_unit):
