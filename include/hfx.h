@@ -84,7 +84,9 @@ typedef struct hfx_stats {
   double flops_fp64, flops_fp32;             /* algorithmic FLOPs of the build */
   double ms_total;                           /* device time of the build, ms */
   double class_flops[HFX_NCLASSES][2];       /* algorithmic FLOPs per class kernel [fp64, fp32] */
-  float class_ms[HFX_NCLASSES][2];           /* class kernel time, ms (HFX_OPT_PROFILE = 1) */
+  float class_ms[HFX_NCLASSES][2];           /* class span, ms (HFX_OPT_PROFILE = 1): first to last of the
+                                                class's launches; with overlapping lists (HFX_OPT_OVERLAP)
+                                                it includes some of its neighbours' work */
   long long bra_runs[HFX_NCLASSES][2];       /* bra pairs with a surviving quartet [fp64, fp32] */
 } hfx_stats;
 
