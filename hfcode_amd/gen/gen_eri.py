@@ -1375,7 +1375,17 @@ class Gen:
         W("    // the step: the run of list[q].x's quartets from q, at most blockDim.x")
         W("    const int bra = d.list[q].x;")
         W("    const int t = q + (int)threadIdx.x;")
-        W("    const bool differ = t >= Q1 || d.list[min(t, Q1 - 1)].x != bra;")
+        W("    const int2 bkg = d.list[min(t, Q1 - 1)];   // the lane's quartet, if the step reaches it")
+        W("    const bool differ = t >= Q1 || bkg.x != bra;")
+        # its ket side (pair record, then P_cd) is loaded here, so that the
+        # dependent loads overlap the step's barriers and the bra's row
+        # changes; a lane past the step keeps these values (it contributes 0)
+        W("    const PairRec& pk = d.pairs[bkg.y];")
+        W("    const int shC = pk.sh%s, shD = pk.sh%s;" % (oc, od))
+        W("    const int aoC = pk.ao%s, aoD = pk.ao%s;" % (oc, od))
+        for c in range(nC):
+            W("    " + " ".join("const T Pcd_%d_%d = PA[(size_t)(aoC + %d) * nbf + aoD + %d];" % (c, dd, c, dd)
+                                for dd in range(nD)))
         W("    const unsigned long long dm = __ballot(differ);")
         W("    if (lane == 0) wrun[threadIdx.x / HFX_WAVE] = dm ? (int)(threadIdx.x - lane) + __ffsll((long long)dm) - 1"
           " : (int)blockDim.x;")
@@ -1407,16 +1417,12 @@ class Gen:
         W("    const bool valid = t < q;")
         W("    const int iq = valid ? t : it;")
         W("    const int slot = iq - d.q0;")
-        W("    const int2 bk = d.list[iq];")
         W("    double vmax = d.vmax[slot];")
         W("    for (int c = 1; c < d.ncombo; ++c) vmax = fmax(vmax, d.vmax[(size_t)c * d.vstride + slot]);")
         W("    const bool act = valid && vmax >= thq;")
         W("    if (__ballot(act) == 0) continue;")
         W("    const T* blk = reinterpret_cast<const T*>(d.eblk) + (size_t)(slot >> 6) * %d + (slot & 63);" % (ne * 64))
-        W("    const PairRec& pb = d.pairs[bk.x];")
-        W("    const PairRec& pk = d.pairs[bk.y];")
-        W("    const int shC = pk.sh%s, shD = pk.sh%s;" % (oc, od))
-        W("    const int aoC = pk.ao%s, aoD = pk.ao%s;" % (oc, od))
+        W("    const PairRec& pb = d.pairs[bra];   // every valid lane's bra")
         W("    const double w = act ? ((pb.sha == pb.shb) ? 0.5 : 1.0) * ((shC == shD) ? 0.5 : 1.0) : 0.0;")
         W("    const double wj = 2.0 * w, wk = -0.5 * w;")
         W("    " + " ".join("T Jcd_%d = 0;" % k for k in range(ncd)))
@@ -1432,9 +1438,6 @@ class Gen:
             def sload(i):
                 W("          " + " ".join("s%d_%d = bp[%d];" % (i, c * nD + dd, lin_i(i, c, dd))
                                           for c in range(nC) for dd in range(nD)))
-            for c in range(nC):
-                for dd in range(nD):
-                    W("      const T Pcd_%d_%d = PA[(size_t)(aoC + %d) * nbf + aoD + %d];" % (c, dd, c, dd))
             if not ki_slice:
                 for i in range(n_i):
                     W("      " + " ".join("T Kic_%d_%d = 0;" % (i, c) for c in range(nC)) + " "
@@ -1537,7 +1540,7 @@ class Gen:
         W("          const int f = f0 + lane;")
         W("          const bool ok = f < TG * %d;" % ncd)
         W("          const int ql = ok ? f / %d : 0, e = ok ? f - ql * %d : 0;" % (ncd, ncd))
-        W("          const int pq = __shfl(bk.y, r * TG + ql);")
+        W("          const int pq = __shfl(bkg.y, r * TG + ql);")
         W("          const int pe = %s;" % ("(e %% %d) * %d + e / %d" % (nD, nC, nD) if self.swap_ket else "e"))
         W("          const double v = ok ? tb[f] : 0.0;")
         W("          if (v != 0.0) atomicAdd(&d.pairbuf[(size_t)pq * HFX_PAIRBUF_STRIDE + pe], v);")
