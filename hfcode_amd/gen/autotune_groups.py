@@ -62,6 +62,7 @@ def _measure_one(lib_path, steps=3):
     dP = torch.from_numpy(np.ascontiguousarray(P)).to(dev)
     dF = torch.empty_like(dP)
     fb.set_profile(True)
+    fb.set_overlap(False)       # lists one after another: a class's span is its own kernels only
     out = {}
     for key, split in (("fp64", 0.0), ("mixed", 1e-5)):
         fb.set_mixed(split)
